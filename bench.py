@@ -1,0 +1,175 @@
+"""Bounce-loop benchmark: converged bounces/s on one MI355X.
+
+Workload (default): the Sinai-like scene -- 4 spheres + 1 ellipsoid, 2,048
+order-4 (5x5 tensor) patches each, 10,240 patches -- and 2^20 rays from the
+z=-8 plane in a 10 degree cone, max 100 bounces, FP64 Newton in the
+`converged` mode, triangle (stage-1) seeds.  One step = one trace of the whole
+ray batch, inputs and outputs resident in HBM.
+
+Prints one JSON line with the throughput, the FP64 roofline of the bounce
+kernel (algorithmic Newton flops / summed kernel time, HIP events on the
+stream the kernels run on) and the CPU oracle timed on a bounded sample of the
+same rays on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "converged bounces/s at ≤1e-10 hit error; mean Newton iters, learned vs tri seed"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rays", type=int, default=1 << 20)
+    ap.add_argument("--max-bounces", type=int, default=100)
+    ap.add_argument("--record-k", type=int, default=-1, help="bounces recorded per ray (-1 = max bounces)")
+    ap.add_argument("--mode", choices=["ref", "converged"], default="converged")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle time budget (0 = skip)")
+    ap.add_argument("--no-cull", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the bounce loop here is built and timed on a single MI355X "
+              "(rays are independent; a multi-GPU run would shard rays and all-reduce hit counts).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from billardballraytracing3d_amd import scenes as S, tracer as T, _lib as L
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    sc, o, d, mb = S.config2(seed=args.seed, n_rays=args.rays)
+    mb = args.max_bounces
+    K = mb if args.record_k < 0 else args.record_k
+    gs = T.GpuScene(sc)
+    t_build = time.time() - t0
+    n = o.shape[0]
+    mode = L.NEWTON_CONVERGED if args.mode == "converged" else L.NEWTON_REF
+    flags = L.FLAG_TIME_KERNELS | (L.FLAG_NO_CULL if args.no_cull else 0)
+
+    to = torch.from_numpy(o).to(dev)
+    td = torch.from_numpy(d).to(dev)
+    outs = dict(status=torch.empty(n, dtype=torch.int8, device=dev),
+                err_kind=torch.empty(n, dtype=torch.int8, device=dev),
+                bounces=torch.empty(n, dtype=torch.int32, device=dev),
+                err_patch=torch.empty(n, dtype=torch.int32, device=dev),
+                final_od=torch.empty((n, 6), dtype=torch.float64, device=dev))
+    if K > 0:
+        outs.update(hit_patch=torch.empty((n, K), dtype=torch.int32, device=dev),
+                    hit_xyz=torch.empty((n, K, 3), dtype=torch.float64, device=dev),
+                    newton_iters=torch.empty((n, K), dtype=torch.uint8, device=dev))
+    ptrs = {k: v.data_ptr() for k, v in outs.items()}
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        return gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=mode, record_k=K, flags=flags,
+                               stream=stream.cuda_stream, block=args.block)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctr_tot = np.zeros(L.NCOUNTERS, np.int64)
+    kms_tot = np.zeros(3)
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        c, k = step()
+        ctr_tot += c
+        kms_tot += k
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    ms_per_step = (te - ts) * 1e3 / args.steps
+    bounces = int(ctr_tot[L.CTR["bounces"]])
+    value = bounces / (te - ts)
+    launches = kms_tot[1]
+    kernel_ms = kms_tot[0]
+    flops = float(ctr_tot[L.CTR["newton_flops"]]) + 27.0 * bounces
+    achieved_tflops = flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    newton_calls = int(ctr_tot[L.CTR["newton_calls"]])
+    newton_iters = int(ctr_tot[L.CTR["newton_iters"]])
+    per_step = {k: int(ctr_tot[v] // args.steps) for k, v in L.CTR.items()}
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "bounces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "sinai-5-obstacles-p4-10240-patches", "rays": n, "patches": sc.n_patches,
+                   "max_bounces": mb, "record_k": K, "newton_mode": args.mode, "seed_mode": "triangle",
+                   "culling": not args.no_cull, "scene_seed": args.seed},
+        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_bounce", "launches_per_step": launches / args.steps,
+                     "avg_launch_ms": kernel_ms / launches if launches else None,
+                     "flops_per_launch": flops / launches if launches else None},
+        "mean_newton_iters": {"triangle": newton_iters / newton_calls if newton_calls else None, "learned": None},
+        "bounces_per_step": bounces // args.steps,
+        "counters_per_step": per_step,
+        "scene_build_s": t_build,
+    }
+
+    if args.cpu_seconds > 0:
+        res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, mode, args.cpu_seconds, gs, K)
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline(sc, o, d, mb, mode, budget_s, gs, K):
+    """The CPU oracle (the reference algorithm: brute-force stage 1, FP64, OpenMP
+    dynamic over rays) on consecutive 2,048-ray chunks of the same batch until
+    the time budget is spent; also checks the GPU's records for those rays."""
+    import numpy as np
+    from oracle import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    osc = O.OracleScene(sc, threads=threads)
+    chunk = 2048
+    t_tot = 0.0
+    bounces = 0
+    rays = 0
+    max_err = 0.0
+    seq_mismatch = 0
+    start = 0
+    while t_tot < budget_s and start < o.shape[0]:
+        sl = slice(start, start + chunk)
+        t0 = time.perf_counter()
+        r = osc.trace(o[sl], d[sl], mb, mode=mode, threads=threads)
+        t_tot += time.perf_counter() - t0
+        bounces += int(r.bounces.sum())
+        rays += r.bounces.shape[0]
+        g = gs.trace(o[sl], d[sl], mb, mode=mode, record_k=mb)
+        kk = min(20, mb)
+        seq_mismatch += int(np.sum(np.any(g.hit_patch[:, :kk] != r.hit_patch[:, :kk], axis=1) |
+                                   (g.status != r.status) | (g.err_kind != r.err_kind) | (g.bounces != r.bounces)))
+        scale = np.maximum(np.linalg.norm(r.hit_xyz, axis=-1), 1.0)
+        max_err = max(max_err, float(np.max(np.linalg.norm(g.hit_xyz - r.hit_xyz, axis=-1) / scale)))
+        start += chunk
+    return {"value": bounces / t_tot if t_tot > 0 else None, "unit": "bounces/s", "cores": threads, "kind": "port",
+            "sample": f"first {rays} rays of the same batch ({bounces} bounces) in {t_tot:.1f} s, oracle "
+                      f"brute-force stage 1, OpenMP dynamic",
+            "gpu_check": {"rays": rays, "max_rel_hit_error": max_err, "rays_with_sequence_mismatch": seq_mismatch}}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,458 @@
+// scene_build.cpp -- host-side scene construction for the bounce-loop library.
+//
+// Per patch this computes what SurfacePatch's constructor computes
+// (include/ScatObjects/SurfacePatch.hpp:125-131): barycentric weights
+// (:335-336, PatchInterpolation.hpp:25-52), centroid and node box (:380-407),
+// the 100x100 coarse grid by first-form barycentric interpolation (:416-436,
+// PatchInterpolation.hpp:54-159), per-cell sagitta and normal and the
+// inflated box (:446-492).  Then the BVH exactly as BVH::build (BVH.hpp:136-189)
+// and the scene diagonal (RayTracingDriver.hpp:386-396).
+//
+// On top of that it builds the FP32 culling pyramid the kernels use to skip
+// coarse cells a ray cannot touch: per cell the box of all 12 corners of the
+// three offset copies (sign 0, +s, -s, SurfacePatch.hpp:287-294), widened by
+// an absolute margin and rounded outward to float; then unions over 3x3 cells
+// (tiles), 3x3 tiles (blocks) and 4x4 blocks (supers).
+//
+// Floating point: compiled with -ffp-contract=off and no -march, so every
+// expression rounds exactly as written; the results are bit-identical to the
+// reference's formulas in the same evaluation order.  The two matrix products
+// of the first-form interpolation sum their inner index upwards from 0.0.
+#include "scene_host.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+#include <thread>
+
+namespace bbrt {
+namespace {
+
+constexpr double kOnNodeTol = 1.0e-12;   // EQUAL_TOL, PatchInterpolation.hpp:21
+
+// std::min / std::max as the reference calls them: (b < a) ? b : a and (a < b) ? b : a.
+inline double lesser(double a, double b) { return (b < a) ? b : a; }
+inline double greater(double a, double b) { return (a < b) ? b : a; }
+
+struct P3 { double x, y, z; };
+inline P3 unit(P3 v) {
+    const double len = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    return { v.x / len, v.y / len, v.z / len };
+}
+
+// One axis of the first-form interpolation operator at `count` points:
+// rows[i][k] = w_k / (x_i - node_k) and scale[i] = prod_k (x_i - node_k), or
+// a one-hot row with scale 1 when x_i is within 1e-12 of a node.
+void first_form_rows(const double* nodes, const double* w, int n, const double* x, int count,
+                     std::vector<double>& rows, std::vector<double>& scale) {
+    rows.assign((size_t)count * n, 0.0);
+    scale.assign(count, 1.0);
+    for (int i = 0; i < count; ++i) {
+        double* r = &rows[(size_t)i * n];
+        double prod = 1.0;
+        int hot = -1;
+        for (int k = 0; k < n; ++k) {
+            const double diff = x[i] - nodes[k];
+            if (std::fabs(diff) < kOnNodeTol) { hot = k; break; }
+            r[k] = w[k] / diff;
+            prod *= diff;
+        }
+        if (hot >= 0) {
+            for (int k = 0; k < n; ++k) r[k] = 0.0;
+            r[hot] = 1.0;
+            scale[i] = 1.0;
+        } else {
+            scale[i] = prod;
+        }
+    }
+}
+
+// Second-form basis at one point (SurfacePatch.hpp:518-538).
+void basis_at(const double* nodes, const double* w, int n, double x, double* b) {
+    for (int i = 0; i < n; ++i) b[i] = 0.0;
+    for (int i = 0; i < n; ++i)
+        if (std::fabs(x - nodes[i]) < kOnNodeTol) { b[i] = 1.0; return; }
+    double denom = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double li = w[i] / (x - nodes[i]);
+        b[i] = li;
+        denom += li;
+    }
+    for (int i = 0; i < n; ++i) b[i] /= denom;
+}
+
+// SurfacePatch.hpp:542-555
+double field_at(const double* lu, const double* lv, const double* f, int imax, int jmax) {
+    double sum = 0.0;
+    for (int i = 0; i < imax; ++i) {
+        const double li = lu[i];
+        if (li == 0.0) continue;
+        double row = 0.0;
+        for (int j = 0; j < jmax; ++j) row += lv[j] * f[(size_t)i * jmax + j];
+        sum += li * row;
+    }
+    return sum;
+}
+
+float round_down(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+    return f;
+}
+float round_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+    return f;
+}
+
+struct PatchWork {
+    const bbrt_patch_set* ps;
+    HostScene* S;
+    std::vector<double> hmax;   // per patch: longest coarse cell diagonal
+};
+
+void prepare_patch(PatchWork& W, int64_t k) {
+    const bbrt_patch_set& ps = *W.ps;
+    HostScene& S = *W.S;
+    const PatchMeta& M = S.meta[k];
+    const int im = M.imax, jm = M.jmax;
+    const double* un = &S.nodes[M.node_off];
+    const double* uw = un + im;
+    const double* vn = uw + im;
+    const double* vw = vn + jm;
+    const double* F = &S.fields[M.field_off];
+    const size_t nn = (size_t)im * jm;
+    (void)ps;
+
+    // centroid and node box (SurfacePatch.hpp:380-407)
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    double lo[3] = { std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
+                     std::numeric_limits<double>::infinity() };
+    double hi[3] = { -std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity(),
+                     -std::numeric_limits<double>::infinity() };
+    for (size_t q = 0; q < nn; ++q) {
+        const double px = F[q], py = F[nn + q], pz = F[2 * nn + q];
+        sx += px; sy += py; sz += pz;
+        lo[0] = lesser(lo[0], px); lo[1] = lesser(lo[1], py); lo[2] = lesser(lo[2], pz);
+        hi[0] = greater(hi[0], px); hi[1] = greater(hi[1], py); hi[2] = greater(hi[2], pz);
+    }
+    const double inv_n = 1.0 / static_cast<double>(nn);
+    double* cent = &S.pcent[3 * k];
+    cent[0] = sx * inv_n; cent[1] = sy * inv_n; cent[2] = sz * inv_n;
+
+    // coarse grid (SurfacePatch.hpp:416-436)
+    double cu[kCoarseN];
+    for (int i = 0; i < kCoarseN; ++i) cu[i] = -1.0 + 2.0 * static_cast<double>(i) / (kCoarseN - 1);
+    std::vector<double> ru, su, rv, sv;
+    first_form_rows(un, uw, im, cu, kCoarseN, ru, su);
+    first_form_rows(vn, vw, jm, cu, kCoarseN, rv, sv);
+    double* G = &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
+    std::vector<double> mid((size_t)kCoarseN * jm);
+    for (int c = 0; c < 3; ++c) {
+        const double* f = F + c * nn;
+        for (int i = 0; i < kCoarseN; ++i)
+            for (int m = 0; m < jm; ++m) {
+                double acc = 0.0;
+                for (int q = 0; q < im; ++q) acc += ru[(size_t)i * im + q] * f[(size_t)q * jm + m];
+                mid[(size_t)i * jm + m] = acc;
+            }
+        for (int i = 0; i < kCoarseN; ++i)
+            for (int j = 0; j < kCoarseN; ++j) {
+                double acc = 0.0;
+                for (int m = 0; m < jm; ++m) acc += mid[(size_t)i * jm + m] * rv[(size_t)j * jm + m];
+                G[((size_t)i * kCoarseN + j) * 3 + c] = acc * (su[i] * sv[j]);
+            }
+    }
+
+    // cell padding (SurfacePatch.hpp:446-492)
+    double* C = &S.cell[(size_t)k * kCells * kCells * 4];
+    double lu[kMaxNodes1D], lv[kMaxNodes1D];
+    double ms = 0.0;
+    bool finite = true;
+    for (int i = 0; i < kCells; ++i)
+        for (int j = 0; j < kCells; ++j) {
+            const double um = 0.5 * (cu[i] + cu[i + 1]);
+            const double vm = 0.5 * (cu[j] + cu[j + 1]);
+            basis_at(un, uw, im, um, lu);
+            basis_at(vn, vw, jm, vm, lv);
+            const P3 tm = { field_at(lu, lv, F, im, jm), field_at(lu, lv, F + nn, im, jm),
+                            field_at(lu, lv, F + 2 * nn, im, jm) };
+            const P3 nm = unit({ field_at(lu, lv, F + 9 * nn, im, jm), field_at(lu, lv, F + 10 * nn, im, jm),
+                                 field_at(lu, lv, F + 11 * nn, im, jm) });
+            const double* p00 = &G[((size_t)i * kCoarseN + j) * 3];
+            const double* p10 = &G[((size_t)(i + 1) * kCoarseN + j) * 3];
+            const double* p01 = &G[((size_t)i * kCoarseN + j + 1) * 3];
+            const double* p11 = &G[((size_t)(i + 1) * kCoarseN + j + 1) * 3];
+            const P3 bm = { 0.25 * (p00[0] + p10[0] + p01[0] + p11[0]), 0.25 * (p00[1] + p10[1] + p01[1] + p11[1]),
+                            0.25 * (p00[2] + p10[2] + p01[2] + p11[2]) };
+            const P3 d = { tm.x - bm.x, tm.y - bm.y, tm.z - bm.z };
+            const double s = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            double* c = &C[((size_t)i * kCells + j) * 4];
+            c[0] = s; c[1] = nm.x; c[2] = nm.y; c[3] = nm.z;
+            ms = greater(ms, s);
+            if (!(std::isfinite(s) && std::isfinite(nm.x) && std::isfinite(nm.y) && std::isfinite(nm.z))) finite = false;
+        }
+    for (size_t q = 0; q < (size_t)kCoarseN * kCoarseN * 3; ++q)
+        if (!std::isfinite(G[q])) finite = false;
+    double* b = &S.pbox[6 * k];
+    b[0] = lo[0] - ms; b[1] = lo[1] - ms; b[2] = lo[2] - ms;
+    b[3] = hi[0] + ms; b[4] = hi[1] + ms; b[5] = hi[2] + ms;
+    S.meta[k].max_sag = ms;
+    S.meta[k].cull = finite ? 1 : 0;
+
+    // longest cell diagonal, for the culling margin
+    double h = 0.0;
+    if (finite)
+        for (int i = 0; i < kCells; ++i)
+            for (int j = 0; j < kCells; ++j) {
+                const double* p00 = &G[((size_t)i * kCoarseN + j) * 3];
+                const double* p11 = &G[((size_t)(i + 1) * kCoarseN + j + 1) * 3];
+                const double* p10 = &G[((size_t)(i + 1) * kCoarseN + j) * 3];
+                const double* p01 = &G[((size_t)i * kCoarseN + j + 1) * 3];
+                double a = 0.0, e = 0.0;
+                for (int c = 0; c < 3; ++c) { a += (p11[c] - p00[c]) * (p11[c] - p00[c]); e += (p10[c] - p01[c]) * (p10[c] - p01[c]); }
+                h = std::max(h, std::sqrt(std::max(a, e)));
+            }
+    W.hmax[k] = h;
+}
+
+// FP32 culling pyramid of one patch, widened by `widen`.
+void build_pyramid(HostScene& S, int64_t k, double widen) {
+    const double* G = &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
+    const double* C = &S.cell[(size_t)k * kCells * kCells * 4];
+    float* bc = &S.box_cell[(size_t)k * kCells * kCells * 6];
+    float* bt = &S.box_tile[(size_t)k * kTileN * kTileN * 6];
+    float* bb = &S.box_block[(size_t)k * kBlockN * kBlockN * 6];
+    float* bs = &S.box_super[(size_t)k * kSuperN * kSuperN * 6];
+    const float inf = std::numeric_limits<float>::infinity();
+    auto empty = [&](float* b) { b[0] = b[1] = b[2] = inf; b[3] = b[4] = b[5] = -inf; };
+    auto merge = [&](float* dst, const float* src) {
+        for (int a = 0; a < 3; ++a) { dst[a] = std::min(dst[a], src[a]); dst[3 + a] = std::max(dst[3 + a], src[3 + a]); }
+    };
+    if (!S.meta[k].cull) {
+        // Never consulted (the kernels test such a patch exhaustively); keep it finite.
+        for (size_t q = 0; q < (size_t)kCells * kCells; ++q) empty(bc + 6 * q);
+        for (size_t q = 0; q < (size_t)kTileN * kTileN; ++q) empty(bt + 6 * q);
+        for (size_t q = 0; q < (size_t)kBlockN * kBlockN; ++q) empty(bb + 6 * q);
+        for (size_t q = 0; q < (size_t)kSuperN * kSuperN; ++q) empty(bs + 6 * q);
+        return;
+    }
+    for (int i = 0; i < kCells; ++i)
+        for (int j = 0; j < kCells; ++j) {
+            const double* c = &C[((size_t)i * kCells + j) * 4];
+            const double* corner[4] = { &G[((size_t)i * kCoarseN + j) * 3], &G[((size_t)(i + 1) * kCoarseN + j) * 3],
+                                        &G[((size_t)i * kCoarseN + j + 1) * 3],
+                                        &G[((size_t)(i + 1) * kCoarseN + j + 1) * 3] };
+            double lo[3] = { std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
+                             std::numeric_limits<double>::infinity() };
+            double hi[3] = { -lo[0], -lo[1], -lo[2] };
+            const double signs[3] = { 0.0, 1.0, -1.0 };
+            for (double sg : signs) {
+                const double d = sg * c[0];
+                for (const double* p : corner) {
+                    // exactly the offset corner the stage-1 test builds (SurfacePatch.hpp:268-270)
+                    const double q[3] = { p[0] + c[1] * d, p[1] + c[2] * d, p[2] + c[3] * d };
+                    for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], q[a]); hi[a] = std::max(hi[a], q[a]); }
+                }
+            }
+            float* b = bc + ((size_t)i * kCells + j) * 6;
+            for (int a = 0; a < 3; ++a) { b[a] = round_down(lo[a] - widen); b[3 + a] = round_up(hi[a] + widen); }
+        }
+    for (int ti = 0; ti < kTileN; ++ti)
+        for (int tj = 0; tj < kTileN; ++tj) {
+            float* b = bt + ((size_t)ti * kTileN + tj) * 6;
+            empty(b);
+            for (int i = 3 * ti; i < 3 * ti + 3; ++i)
+                for (int j = 3 * tj; j < 3 * tj + 3; ++j) merge(b, bc + ((size_t)i * kCells + j) * 6);
+        }
+    for (int bi = 0; bi < kBlockN; ++bi)
+        for (int bj = 0; bj < kBlockN; ++bj) {
+            float* b = bb + ((size_t)bi * kBlockN + bj) * 6;
+            empty(b);
+            for (int ti = 3 * bi; ti < 3 * bi + 3; ++ti)
+                for (int tj = 3 * bj; tj < 3 * bj + 3; ++tj) merge(b, bt + ((size_t)ti * kTileN + tj) * 6);
+        }
+    for (int si = 0; si < kSuperN; ++si)
+        for (int sj = 0; sj < kSuperN; ++sj) {
+            float* b = bs + ((size_t)si * kSuperN + sj) * 6;
+            empty(b);
+            for (int bi = kBlocksPerSuper * si; bi < std::min(kBlockN, kBlocksPerSuper * (si + 1)); ++bi)
+                for (int bj = kBlocksPerSuper * sj; bj < std::min(kBlockN, kBlocksPerSuper * (sj + 1)); ++bj)
+                    merge(b, bb + ((size_t)bi * kBlockN + bj) * 6);
+        }
+}
+
+// BVH::build (BVH.hpp:136-189) over patch indices.
+struct TreeBuilder {
+    const std::vector<double>& box;    // [n][6]
+    const std::vector<double>& cent;   // [n][3]
+    int leaf_threshold;
+    std::vector<BvhNode>& nodes;
+    std::vector<int32_t>& leaf;
+
+    int32_t build(std::vector<int32_t> objs) {
+        const int32_t me = (int32_t)nodes.size();
+        nodes.emplace_back();
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = box[6 * objs.front() + a]; hi[a] = box[6 * objs.front() + 3 + a]; }
+        for (size_t i = 1; i < objs.size(); ++i)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = lesser(lo[a], box[6 * objs[i] + a]);
+                hi[a] = greater(hi[a], box[6 * objs[i] + 3 + a]);
+            }
+        if ((int)objs.size() <= leaf_threshold) {
+            BvhNode& nd = nodes[me];
+            for (int a = 0; a < 3; ++a) { nd.lo[a] = lo[a]; nd.hi[a] = hi[a]; }
+            nd.right = -1; nd.start = (int32_t)leaf.size(); nd.count = (int32_t)objs.size(); nd.axis = 0;
+            for (int32_t o : objs) leaf.push_back(o);
+            return me;
+        }
+        double cmin[3], cmax[3];
+        for (int a = 0; a < 3; ++a) cmin[a] = cmax[a] = cent[3 * objs.front() + a];
+        for (int32_t o : objs)
+            for (int a = 0; a < 3; ++a) {
+                cmin[a] = lesser(cmin[a], cent[3 * o + a]);
+                cmax[a] = greater(cmax[a], cent[3 * o + a]);
+            }
+        double spread[3] = { cmax[0] - cmin[0], cmax[1] - cmin[1], cmax[2] - cmin[2] };
+        const int axis = (int)(std::max_element(spread, spread + 3) - spread);
+        auto mid = objs.begin() + objs.size() / 2;
+        const std::vector<double>& c = cent;
+        std::nth_element(objs.begin(), mid, objs.end(),
+                         [axis, &c](int32_t a, int32_t b) { return c[3 * a + axis] < c[3 * b + axis]; });
+        std::vector<int32_t> left(objs.begin(), mid), right(mid, objs.end());
+        objs.clear(); objs.shrink_to_fit();
+        build(std::move(left));
+        const int32_t r = build(std::move(right));
+        BvhNode& nd = nodes[me];
+        for (int a = 0; a < 3; ++a) { nd.lo[a] = lo[a]; nd.hi[a] = hi[a]; }
+        nd.right = r; nd.start = 0; nd.count = 0; nd.axis = axis;
+        return me;
+    }
+};
+
+template <class F>
+void parallel_for(int64_t n, int threads, F&& fn) {
+    if (threads <= 1 || n < 64) { for (int64_t k = 0; k < n; ++k) fn(k); return; }
+    std::atomic<int64_t> next{ 0 };
+    std::vector<std::thread> pool;
+    std::exception_ptr err = nullptr;
+    std::atomic<bool> failed{ false };
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&]() {
+            try {
+                for (;;) {
+                    const int64_t k = next.fetch_add(16);
+                    if (k >= n || failed.load()) break;
+                    for (int64_t q = k; q < std::min(n, k + 16); ++q) fn(q);
+                }
+            } catch (...) {
+                if (!failed.exchange(true)) err = std::current_exception();
+            }
+        });
+    for (auto& th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
+
+}  // namespace
+
+void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads) {
+    const int64_t n = ps.n_patches;
+    if (n <= 0) throw std::runtime_error("bbrt_scene_create: no patches");
+    if (!ps.imax || !ps.jmax || !ps.u_off || !ps.v_off || !ps.f_off || !ps.unodes || !ps.vnodes || !ps.fields)
+        throw std::runtime_error("bbrt_scene_create: null array in patch set");
+    S.n = n;
+    S.meta.resize(n);
+    int64_t node_total = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const int im = ps.imax[k], jm = ps.jmax[k];
+        if (im < 2 || jm < 2 || im > kMaxNodes1D || jm > kMaxNodes1D)
+            throw std::runtime_error("bbrt_scene_create: patch " + std::to_string(k) + " has imax/jmax outside [2, 16]");
+        if (ps.u_off[k + 1] - ps.u_off[k] != im || ps.v_off[k + 1] - ps.v_off[k] != jm ||
+            ps.f_off[k + 1] - ps.f_off[k] != (int64_t)im * jm)
+            throw std::runtime_error("bbrt_scene_create: inconsistent offsets at patch " + std::to_string(k));
+        PatchMeta& M = S.meta[k];
+        M.imax = im; M.jmax = jm;
+        M.node_off = node_total;
+        M.field_off = 12 * ps.f_off[k];
+        M.max_sag = 0.0; M.cull = 0; M.pad0 = 0; M.pad1 = 0.0;
+        node_total += 2 * (im + jm);
+    }
+    S.nodes.resize(node_total);
+    const int64_t field_total = 12 * ps.f_off[n];
+    S.fields.assign(ps.fields, ps.fields + field_total);
+    for (int64_t k = 0; k < n; ++k) {
+        const PatchMeta& M = S.meta[k];
+        double* dst = &S.nodes[M.node_off];
+        const double* u = ps.unodes + ps.u_off[k];
+        const double* v = ps.vnodes + ps.v_off[k];
+        auto weights = [](const double* x, int m, double* w) {   // PatchInterpolation.hpp:25-52
+            for (int j = 0; j < m; ++j) {
+                double p = 1.0;
+                for (int i = 0; i < m; ++i)
+                    if (i != j) p *= (x[j] - x[i]);
+                w[j] = 1.0 / p;
+            }
+        };
+        for (int i = 0; i < M.imax; ++i) dst[i] = u[i];
+        weights(u, M.imax, dst + M.imax);
+        for (int j = 0; j < M.jmax; ++j) dst[2 * M.imax + j] = v[j];
+        weights(v, M.jmax, dst + 2 * M.imax + M.jmax);
+    }
+    S.cxyz.assign((size_t)n * kCoarseN * kCoarseN * 3, 0.0);
+    S.cell.assign((size_t)n * kCells * kCells * 4, 0.0);
+    S.pbox.assign((size_t)n * 6, 0.0);
+    S.pcent.assign((size_t)n * 3, 0.0);
+    PatchWork W{ &ps, &S, std::vector<double>(n, 0.0) };
+    parallel_for(n, threads, [&](int64_t k) { prepare_patch(W, k); });
+
+    // scene diagonal and largest sagitta (RayTracingDriver.hpp:386-396)
+    double lo[3] = { S.pbox[0], S.pbox[1], S.pbox[2] }, hi[3] = { S.pbox[3], S.pbox[4], S.pbox[5] };
+    double msag = 0.0, scale = 1.0, hmax = 0.0;
+    for (int64_t k = 0; k < n; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = lesser(lo[a], S.pbox[6 * k + a]);
+            hi[a] = greater(hi[a], S.pbox[6 * k + 3 + a]);
+        }
+        msag = greater(msag, S.meta[k].max_sag);
+        if (S.meta[k].cull) {
+            for (int a = 0; a < 6; ++a) scale = std::max(scale, std::fabs(S.pbox[6 * k + a]));
+            hmax = std::max(hmax, W.hmax[k]);
+        }
+    }
+    const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    S.diag = std::sqrt(dx * dx + dy * dy + dz * dz);
+    S.max_patch_sag = msag;
+    S.scale = scale;
+    // Margin of the FP32 culling boxes.  It must exceed (a) the FP32 slab-test
+    // error for rays starting within 8*scale of the origin, about
+    // 4 * 2^-24 * 20 * scale, and (b) how far outside its triangle a hit that
+    // Moller-Trumbore accepts can lie, at most about 2^-52 * |o - v0| * h^2 / 1e-9
+    // for cell diagonal h (the 1e-9 is the determinant guard).  1e-5 * scale *
+    // max(1, h^2) exceeds both by more than an order of magnitude.
+    S.cull_widen = 1e-5 * scale * std::max(1.0, hmax * hmax);
+
+    S.box_cell.assign((size_t)n * kCells * kCells * 6, 0.0f);
+    S.box_tile.assign((size_t)n * kTileN * kTileN * 6, 0.0f);
+    S.box_block.assign((size_t)n * kBlockN * kBlockN * 6, 0.0f);
+    S.box_super.assign((size_t)n * kSuperN * kSuperN * 6, 0.0f);
+    parallel_for(n, threads, [&](int64_t k) { build_pyramid(S, k, S.cull_widen); });
+
+    // BVH (RayTracingDriver.hpp:375-384 -> BVH(objects), leaf threshold 4)
+    build_bvh(S.pbox, S.pcent, 4, S.bvh, S.leaf);
+}
+
+void build_bvh(const std::vector<double>& box, const std::vector<double>& cent, int leaf_threshold,
+               std::vector<BvhNode>& nodes, std::vector<int32_t>& leaf) {
+    const int64_t n = (int64_t)cent.size() / 3;
+    nodes.clear(); leaf.clear();
+    nodes.reserve(2 * n); leaf.reserve(n);
+    std::vector<int32_t> objs(n);
+    for (int64_t k = 0; k < n; ++k) objs[k] = (int32_t)k;
+    TreeBuilder tb{ box, cent, leaf_threshold, nodes, leaf };
+    tb.build(std::move(objs));
+}
+
+}  // namespace bbrt

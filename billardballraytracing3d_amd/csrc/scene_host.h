@@ -1,0 +1,76 @@
+// scene_host.h -- host-side scene representation shared by the scene builder
+// (scene_build.cpp) and the device runtime (bbrt_hip.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bbrt.h"
+
+namespace bbrt {
+
+constexpr int kCoarseN = 100;                         // SurfacePatch.hpp:117
+constexpr int kCells = kCoarseN - 1;                  // 99 cells per side
+constexpr int kTileN = 33, kBlockN = 11, kSuperN = 3; // culling pyramid: 3x3 cells/tile, 3x3 tiles/block,
+constexpr int kBlocksPerSuper = 4;                    // 4x4 blocks per super (last row/col: 3)
+constexpr int kMaxNodes1D = 16;                       // largest imax / jmax the kernels accept
+
+enum : int8_t { kEscaped = BBRT_ESCAPED, kTrapped = BBRT_TRAPPED, kError = BBRT_ERROR };
+enum : int {
+    kErrNone = BBRT_ERR_NONE, kErrSingular = BBRT_ERR_SINGULAR, kErrNoConv = BBRT_ERR_NOCONV,
+    kErrDomain = BBRT_ERR_DOMAIN, kErrBackface = BBRT_ERR_BACKFACE, kErrZeroDir = BBRT_ERR_ZERO_DIR,
+    kErrStack = BBRT_ERR_STACK
+};
+enum : int {
+    kCtrRays = BBRT_CTR_RAYS, kCtrBounces = BBRT_CTR_BOUNCES, kCtrBvh = BBRT_CTR_BVH_NODE_TESTS,
+    kCtrCand = BBRT_CTR_CANDIDATES, kCtrCull = BBRT_CTR_CULL_BOX_TESTS, kCtrMt = BBRT_CTR_MT_TESTS,
+    kCtrNewtonCalls = BBRT_CTR_NEWTON_CALLS, kCtrNewtonIters = BBRT_CTR_NEWTON_ITERS,
+    kCtrNewtonFlops = BBRT_CTR_NEWTON_FLOPS, kCtrPolishFail = BBRT_CTR_POLISH_FAIL, kCtrLaunches = BBRT_CTR_LAUNCHES
+};
+
+// Per-patch record, 48 bytes.
+struct PatchMeta {
+    int32_t imax, jmax;
+    int64_t node_off;    // into nodes[]: u nodes, u weights, v nodes, v weights
+    int64_t field_off;   // into fields[]: 12 * f_off[k]
+    double max_sag;
+    int32_t cull;        // 1 when every coarse value is finite (culling pyramid usable)
+    int32_t pad0;
+    double pad1;
+};
+
+// BVH node, 64 bytes, the reference's depth-first layout (BVH.hpp:56-62).
+struct BvhNode {
+    double lo[3], hi[3];
+    int32_t right, start, count, axis;
+};
+
+struct HostScene {
+    int64_t n = 0;
+    std::vector<PatchMeta> meta;
+    std::vector<double> nodes;      // per patch: un[imax] uw[imax] vn[jmax] vw[jmax]
+    std::vector<double> fields;     // 12 * total nodes
+    std::vector<double> cxyz;       // [n][100*100][3]
+    std::vector<double> cell;       // [n][99*99][4] = sagitta, normal xyz
+    std::vector<float> box_cell;    // [n][99*99][6]
+    std::vector<float> box_tile;    // [n][33*33][6]
+    std::vector<float> box_block;   // [n][11*11][6]
+    std::vector<float> box_super;   // [n][3*3][6]
+    std::vector<double> pbox;       // [n][6] inflated patch boxes
+    std::vector<double> pcent;      // [n][3]
+    std::vector<BvhNode> bvh;
+    std::vector<int32_t> leaf;
+    double diag = 0.0, max_patch_sag = 0.0;
+    double scale = 1.0;             // max(1, |coordinate|) over all patch boxes
+    double cull_widen = 0.0;        // absolute widening of the FP32 culling boxes
+};
+
+// Builds everything above from a patch set; throws std::runtime_error on bad input.
+void build_host_scene(const bbrt_patch_set& ps, HostScene& out, int threads);
+
+// BVH::build (BVH.hpp:136-189) over objects with boxes [n][6] and centroids [n][3].
+void build_bvh(const std::vector<double>& box, const std::vector<double>& cent, int leaf_threshold,
+               std::vector<BvhNode>& nodes, std::vector<int32_t>& leaf);
+
+}  // namespace bbrt

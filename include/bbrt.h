@@ -1,0 +1,165 @@
+/* bbrt.h -- C ABI of libbbrt_hip.so, the MI355X (gfx950) billiard bounce loop.
+ *
+ * The library replaces the reference's per-ray trace loop,
+ *   RayTracingDriver::trace_ray        (include/RayTracingDriver.hpp:172-242)
+ * as driven by
+ *   RayTracingDriver::run              (include/RayTracingDriver.hpp:143-150),
+ *   PlaneSourceDriver::run             (include/PlaneSourceDriver.hpp:79-109),
+ *   BounceMapDriver::run               (include/BounceMapDriver.hpp:202-238),
+ * including everything below it: BVH::intersect (include/BVH.hpp:75-128),
+ * SurfacePatch::intersect (include/ScatObjects/SurfacePatch.hpp:253-303),
+ * moller_trumbore (include/Utils/MollerTrumbore.hpp:32-64) and newton_refine
+ * (SurfacePatch.hpp:593-660).  Scene construction (SurfacePatch's constructor,
+ * SurfacePatch.hpp:125-131, and RayTracingDriver::build_bvh,
+ * RayTracingDriver.hpp:375-384) happens in bbrt_scene_create.
+ *
+ * Conventions: every call returns 0 on success or a negative code, with a
+ * message in bbrt_last_error() (thread-local).  No C++ exception crosses the
+ * ABI.  A scene handle owns all of its device memory and one HIP stream; a
+ * handle is not reentrant, distinct handles may be used from distinct threads.
+ */
+#ifndef BBRT_H
+#define BBRT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bbrt_scene_s* bbrt_scene;
+
+/* Tensor-product patches in patch-index order.  The reference's patch index is
+ * the lexicographic order of the patch file names (RayTracingDriver.hpp:361,
+ * :379-382); callers that load files pass them in that order.
+ *   imax, jmax   [n]     node counts per patch (SurfacePatch::Data, SurfacePatch.hpp:60-61)
+ *   u_off, v_off [n+1]   offsets of patch k's u and v nodes in unodes / vnodes
+ *   f_off        [n+1]   node offsets: patch k's 12 fields start at fields + 12*f_off[k],
+ *                        laid out [field][i*jmax + j] in the order
+ *                        x y z dxdu dydu dzdu dxdv dydv dzdv nuX nuY nuZ
+ *                        (SurfacePatch.hpp:74-80; dS and mask are not read). */
+typedef struct {
+    int64_t n_patches;
+    const int32_t* imax;
+    const int32_t* jmax;
+    const int64_t* u_off;
+    const int64_t* v_off;
+    const int64_t* f_off;
+    const double* unodes;
+    const double* vnodes;
+    const double* fields;
+} bbrt_patch_set;
+
+enum { BBRT_NEWTON_REF = 0,          /* the reference stopping rule (step < 1e-3), SurfacePatch.hpp:598-630 */
+       BBRT_NEWTON_CONVERGED = 1 };  /* same classification, then FP64 polish to step < 1e-13*max(1,|t|) */
+enum { BBRT_SEED_TRIANGLE = 0, BBRT_SEED_LEARNED = 1 };
+/* RayPath::Status, RayTracingDriver.hpp:69 */
+enum { BBRT_ESCAPED = 0, BBRT_TRAPPED = 1, BBRT_ERROR = 2 };
+/* Error kinds: the throw sites SurfacePatch.hpp:618-622 (singular), :632-637
+ * (no convergence), :639-646 (domain), RayTracingDriver.hpp:220-226 (back
+ * face), :176-181 (zero direction); STACK = traversal stack overflow. */
+enum { BBRT_ERR_NONE = 0, BBRT_ERR_SINGULAR = 1, BBRT_ERR_NOCONV = 2, BBRT_ERR_DOMAIN = 3,
+       BBRT_ERR_BACKFACE = 4, BBRT_ERR_ZERO_DIR = 5, BBRT_ERR_STACK = 6 };
+enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute force (all 58,806 tests) */
+       BBRT_FLAG_COUNT_DIRECT_HIT = 2u, /* hit_count also counts each ray's first hit (PlaneSourceDriver.hpp:101) */
+       BBRT_FLAG_TIME_KERNELS = 4u };   /* record HIP events around every bounce-kernel launch */
+
+typedef struct {
+    int32_t newton_mode;   /* BBRT_NEWTON_* */
+    int32_t seed_mode;     /* BBRT_SEED_* (only TRIANGLE is implemented) */
+    int32_t max_bounces;   /* run(max_bounces), RayTracingDriver.hpp:143 */
+    int32_t record_k;      /* bounces recorded per ray in hit_patch / hit_xyz / newton_iters */
+    uint32_t flags;        /* BBRT_FLAG_* */
+    int32_t block;         /* threads per block of the bounce kernel; 0 = library default */
+} bbrt_params;
+
+/* Outputs; any pointer may be NULL.  Host memory for bbrt_trace, device
+ * memory for bbrt_trace_device.
+ *   status, err_kind       [n] int8    RayPath::status and the error kind
+ *   bounces                [n] int32   RayPath::bounce_count
+ *   err_patch              [n] int32   patch whose intersection failed, -1 otherwise
+ *   hit_patch              [n][record_k] int32  RayPath::hit_objects as patch indices, -1 padded
+ *   hit_xyz                [n][record_k][3] double  RayPath::points[1..]
+ *   newton_iters           [n][record_k] uint8  Newton iterations of each recorded hit
+ *   final_od               [n][6] double  origin and unit direction when the ray stopped
+ *                          (an escaped ray's tail point is origin + dir * scene diagonal,
+ *                          RayTracingDriver.hpp:198-203)
+ *   hit_count              [n_patches] int64  PlaneSourceDriver hit counts (accumulated,
+ *                          not cleared), first hit skipped unless BBRT_FLAG_COUNT_DIRECT_HIT */
+typedef struct {
+    int8_t* status;
+    int8_t* err_kind;
+    int32_t* bounces;
+    int32_t* err_patch;
+    int32_t* hit_patch;
+    double* hit_xyz;
+    uint8_t* newton_iters;
+    double* final_od;
+    int64_t* hit_count;
+} bbrt_outputs;
+
+/* Work counters returned by the trace calls (int64[BBRT_NCOUNTERS]). */
+enum { BBRT_CTR_RAYS = 0, BBRT_CTR_BOUNCES = 1, BBRT_CTR_BVH_NODE_TESTS = 2, BBRT_CTR_CANDIDATES = 3,
+       BBRT_CTR_CULL_BOX_TESTS = 4, BBRT_CTR_MT_TESTS = 5, BBRT_CTR_NEWTON_CALLS = 6, BBRT_CTR_NEWTON_ITERS = 7,
+       BBRT_CTR_NEWTON_FLOPS = 8, BBRT_CTR_POLISH_FAIL = 9, BBRT_CTR_LAUNCHES = 10, BBRT_NCOUNTERS = 16 };
+
+/* Builds a scene: per patch the barycentric weights, centroid, node box,
+ * 100x100 coarse grid, per-cell sagitta and normal and the inflated box
+ * (SurfacePatch.hpp:125-131, :335-336, :380-492); then the median-split BVH
+ * with leaf threshold 4 (BVH.hpp:136-189) and the scene diagonal
+ * (RayTracingDriver.hpp:386-396); then uploads everything to the device. */
+int bbrt_scene_create(const bbrt_patch_set* patches, bbrt_scene* out);
+void bbrt_scene_destroy(bbrt_scene scene);
+
+/* Scene scalars: scene diagonal, largest patch sagitta, BVH node count, patch
+ * count and device bytes held. Any pointer may be NULL. */
+int bbrt_scene_info(bbrt_scene scene, double* diag, double* max_patch_sagitta, int64_t* n_nodes,
+                    int64_t* n_patches, int64_t* device_bytes);
+/* Per patch: inflated box [n][6] (min xyz, max xyz), centroid [n][3], max sagitta [n]. */
+int bbrt_scene_patch_info(bbrt_scene scene, double* box, double* centroid, double* max_sagitta);
+/* The flattened BVH in the reference's depth-first layout (BVH.hpp:56-62):
+ * node boxes [m][6], right child (-1 = leaf), leaf object start and count,
+ * split axis, and the leaf object array [n_patches]. */
+int bbrt_scene_bvh(bbrt_scene scene, double* node_box, int32_t* right, int32_t* start, int32_t* count,
+                   uint8_t* axis, int32_t* leaf_objects);
+/* Coarse grid of patch k: vertices [100*100][3], cell sagitta [99*99], cell normals [99*99][3]. */
+int bbrt_scene_coarse(bbrt_scene scene, int64_t k, double* xyz, double* sagitta, double* cell_normal);
+
+/* Traces n rays (host memory in and out) -- RayTracingDriver::set_rays + run
+ * (RayTracingDriver.hpp:119-150) for the whole batch.  counters may be NULL. */
+int bbrt_trace(bbrt_scene scene, const double* origins, const double* dirs, int64_t n, const bbrt_params* params,
+               const bbrt_outputs* out, int64_t* counters);
+
+/* Same with origins, dirs and outputs in device memory, on `stream` (a
+ * hipStream_t; NULL = the scene's own stream).  Returns after the work is
+ * done.  kernel_ms (may be NULL) receives [0] the summed duration of the
+ * bounce-kernel launches (BBRT_FLAG_TIME_KERNELS), [1] the launch count,
+ * [2] the wall time of the whole call on the stream, in ms. */
+int bbrt_trace_device(bbrt_scene scene, const double* origins, const double* dirs, int64_t n,
+                      const bbrt_params* params, const bbrt_outputs* out, int64_t* counters, void* stream,
+                      double* kernel_ms);
+
+/* Host-only scene preparation (no device needed): what bbrt_scene_create
+ * computes per patch.  box [n][6], centroid [n][3], max_sagitta [n],
+ * coarse_xyz [n][100*100][3], cell [n][99*99][4] (sagitta, normal xyz); any
+ * output may be NULL.  Mirrors SurfacePatch's constructor
+ * (SurfacePatch.hpp:125-131, :380-492). */
+int bbrt_prepare_patches(const bbrt_patch_set* patches, double* box, double* centroid, double* max_sagitta,
+                         double* coarse_xyz, double* cell);
+
+/* Host-only BVH build over arbitrary objects, BVH::BVH(objects,
+ * leaf_threshold) (BVH.hpp:67-73, :136-189): boxes [n][6], centroids [n][3].
+ * Outputs are sized [2n] (nodes) and [n] (leaf objects); returns the node
+ * count, or -1 on error. */
+int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int32_t leaf_threshold,
+                       double* node_box, int32_t* right, int32_t* start, int32_t* count, uint8_t* axis,
+                       int32_t* leaf_objects);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* bbrt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBRT_H */

@@ -1,0 +1,88 @@
+"""GPU parity: the HIP bounce loop against the CPU FP64 oracle on identical inputs.
+
+Integer outputs (status, error kind, bounce count, failing patch, per-bounce
+patch sequence, Newton iteration counts) and the work counters that the
+reference's control flow fixes (BVH node tests, stage-1 candidates, Newton
+calls and iterations) must match exactly.  Hit points and final ray states
+are compared bit-for-bit: both sides evaluate the same FP64 expressions in
+the same order without contraction, so the tolerance is zero.
+"""
+import numpy as np
+import pytest
+
+from billardballraytracing3d_amd import scenes as S
+from billardballraytracing3d_amd import tracer as T
+from billardballraytracing3d_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+_CACHE = {}
+
+
+def _scene(name):
+    if name not in _CACHE:
+        if name == "c1":
+            sc, o, d, mb = S.config1()
+        elif name == "c2s":
+            sc, o, d, mb = S.config2(n_rays=2048)
+        else:
+            raise KeyError(name)
+        _CACHE[name] = (sc, o, d, mb)
+    return _CACHE[name]
+
+
+def _oracle(name, mode, sc, o, d, mb):
+    from oracle import oracle as O
+    key = (name, "oracle", mode)
+    if key not in _CACHE:
+        osc = O.OracleScene(sc)
+        _CACHE[key] = osc.trace(o, d, mb, mode=mode)
+    return _CACHE[key]
+
+
+def _gpu_scene(name, sc):
+    key = (name, "gpu")
+    if key not in _CACHE:
+        _CACHE[key] = T.GpuScene(sc)
+    return _CACHE[key]
+
+
+def assert_same(g, r, label):
+    np.testing.assert_array_equal(g.status, r.status, err_msg=label + " status")
+    np.testing.assert_array_equal(g.err_kind, r.err_kind, err_msg=label + " err_kind")
+    np.testing.assert_array_equal(g.bounces, r.bounces, err_msg=label + " bounces")
+    np.testing.assert_array_equal(g.err_patch, r.err_patch, err_msg=label + " err_patch")
+    np.testing.assert_array_equal(g.hit_patch, r.hit_patch, err_msg=label + " hit_patch")
+    np.testing.assert_array_equal(g.iters, r.iters, err_msg=label + " newton iters")
+    np.testing.assert_array_equal(g.hit_xyz, r.hit_xyz, err_msg=label + " hit_xyz (bit-exact)")
+    np.testing.assert_array_equal(g.final_od, r.final_od, err_msg=label + " final origin/direction")
+
+
+@pytest.mark.parametrize("name", ["c1", "c2s"])
+@pytest.mark.parametrize("mode", [L.NEWTON_REF, L.NEWTON_CONVERGED])
+def test_trace_matches_oracle(name, mode):
+    sc, o, d, mb = _scene(name)
+    print(f"[parity] scene={name} mode={mode} rays={o.shape[0]} patches={sc.n_patches}", flush=True)
+    r = _oracle(name, mode, sc, o, d, mb)
+    g = _gpu_scene(name, sc).trace(o, d, mb, mode=mode)
+    assert_same(g, r, f"{name}/mode{mode}")
+    # counters the reference's control flow fixes: [calls, iters, candidates, mt, box tests, polish fail]
+    assert g.counter("newton_calls") == r.counters[0]
+    assert g.counter("newton_iters") == r.counters[1]
+    assert g.counter("candidates") == r.counters[2]
+    assert g.counter("bvh_node_tests") == r.counters[4]
+    assert g.counter("polish_fail") == r.counters[5]
+    assert g.counter("bounces") == int(r.bounces.sum())
+    assert g.counter("rays") == o.shape[0]
+
+
+@pytest.mark.parametrize("name", ["c1", "c2s"])
+def test_culling_equals_brute_force(name):
+    sc, o, d, mb = _scene(name)
+    gs = _gpu_scene(name, sc)
+    print(f"[cull] scene={name}", flush=True)
+    a = gs.trace(o, d, mb, mode=L.NEWTON_CONVERGED)
+    b = gs.trace(o, d, mb, mode=L.NEWTON_CONVERGED, flags=L.FLAG_NO_CULL)
+    assert_same(a, b, name + " culled vs brute force")
+    assert b.counter("mt_tests") == 6 * 99 * 99 * b.counter("candidates")
+    assert a.counter("mt_tests") < b.counter("mt_tests")
