@@ -38,7 +38,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle time budget (0 = skip)")
     ap.add_argument("--no-cull", action="store_true")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--waves-per-eu", type=int, default=0)
+    ap.add_argument("--wavefront", action="store_true", help="per-bounce launches instead of the persistent kernel")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a device copy of this many MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe_mb > 0:
+        copy_probe(args.copy_probe_mb)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -60,7 +67,7 @@ def main():
     t_build = time.time() - t0
     n = o.shape[0]
     mode = L.NEWTON_CONVERGED if args.mode == "converged" else L.NEWTON_REF
-    flags = L.FLAG_TIME_KERNELS | (L.FLAG_NO_CULL if args.no_cull else 0)
+    flags = L.FLAG_TIME_KERNELS | (L.FLAG_NO_CULL if args.no_cull else 0) | (L.FLAG_WAVEFRONT if args.wavefront else 0)
 
     to = torch.from_numpy(o).to(dev)
     td = torch.from_numpy(d).to(dev)
@@ -78,7 +85,7 @@ def main():
 
     def step():
         return gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=mode, record_k=K, flags=flags,
-                               stream=stream.cuda_stream, block=args.block)
+                               stream=stream.cuda_stream, block=args.block, waves_per_eu=args.waves_per_eu)
 
     for _ in range(args.warmup):
         step()
@@ -118,10 +125,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "sinai-5-obstacles-p4-10240-patches", "rays": n, "patches": sc.n_patches,
                    "max_bounces": mb, "record_k": K, "newton_mode": args.mode, "seed_mode": "triangle",
-                   "culling": not args.no_cull, "scene_seed": args.seed},
+                   "culling": not args.no_cull, "scene_seed": args.seed,
+                   "schedule": "wavefront" if args.wavefront else "persistent", "block": args.block or 256,
+                   "waves_per_eu": args.waves_per_eu or 2},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_bounce", "launches_per_step": launches / args.steps,
+                     "kernel": "k_bounce" if args.wavefront else "k_persist", "launches_per_step": launches / args.steps,
                      "avg_launch_ms": kernel_ms / launches if launches else None,
                      "flops_per_launch": flops / launches if launches else None},
         "mean_newton_iters": {"triangle": newton_iters / newton_calls if newton_calls else None, "learned": None},
@@ -133,6 +142,18 @@ def main():
     if args.cpu_seconds > 0:
         res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, mode, args.cpu_seconds, gs, K)
     print(json.dumps(res), flush=True)
+
+
+def copy_probe(mb):
+    """A plain device-to-device copy of known size, for scaling the PMC byte counters."""
+    import torch
+    n = mb * (1 << 20) // 8
+    x = torch.arange(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}), flush=True)
 
 
 def cpu_baseline(sc, o, d, mb, mode, budget_s, gs, K):

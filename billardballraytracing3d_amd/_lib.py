@@ -18,7 +18,7 @@ NEWTON_REF, NEWTON_CONVERGED = 0, 1
 SEED_TRIANGLE, SEED_LEARNED = 0, 1
 ESCAPED, TRAPPED, ERROR = 0, 1, 2
 ERR_NONE, ERR_SINGULAR, ERR_NOCONV, ERR_DOMAIN, ERR_BACKFACE, ERR_ZERO_DIR, ERR_STACK = range(7)
-FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS = 1, 2, 4
+FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS, FLAG_WAVEFRONT = 1, 2, 4, 8
 NCOUNTERS = 16
 CTR = dict(rays=0, bounces=1, bvh_node_tests=2, candidates=3, cull_box_tests=4, mt_tests=5, newton_calls=6,
            newton_iters=7, newton_flops=8, polish_fail=9, launches=10)
@@ -42,13 +42,13 @@ class PatchSet(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("newton_mode", C.c_int32), ("seed_mode", C.c_int32), ("max_bounces", C.c_int32),
-                ("record_k", C.c_int32), ("flags", C.c_uint32), ("block", C.c_int32)]
+                ("record_k", C.c_int32), ("flags", C.c_uint32), ("block", C.c_int32), ("waves_per_eu", C.c_int32)]
 
 
 class Outputs(C.Structure):
     _fields_ = [("status", C.c_void_p), ("err_kind", C.c_void_p), ("bounces", C.c_void_p), ("err_patch", C.c_void_p),
                 ("hit_patch", C.c_void_p), ("hit_xyz", C.c_void_p), ("newton_iters", C.c_void_p),
-                ("final_od", C.c_void_p), ("hit_count", C.c_void_p)]
+                ("final_od", C.c_void_p), ("hit_count", C.c_void_p), ("ray_work", C.c_void_p)]
 
 
 class BBRTError(RuntimeError):

@@ -43,6 +43,7 @@ namespace {
 thread_local std::string g_last_error;
 
 constexpr int kStackCap = 48;
+constexpr int kDefaultWavesPerEu = 2;         // persistent-kernel occupancy target (autotuned per scene)
 constexpr double kMtEps = 1e-9;               // MollerTrumbore.hpp:33
 constexpr double kOnNodeTol = 1.0e-12;        // PatchInterpolation.hpp:21
 constexpr int kNewtonMaxIters = 20;           // SurfacePatch.hpp:121
@@ -86,6 +87,7 @@ struct Outs {
     uint8_t* newton_iters;
     double* final_od;
     unsigned long long* hit_count;
+    uint32_t* ray_work;
 };
 
 struct V3 { double x, y, z; };
@@ -205,36 +207,59 @@ __device__ void stage1_brute(const double* G, const double* C, const V3& o, cons
         for (int j = 0; j < bbrt::kCells; ++j) cell_tests(G, C, i, j, o, d, best, c);
 }
 
-// Stage 1 through the culling pyramid.
+// Loads one aligned group of NB boxes (NF floats) and returns the mask of the
+// boxes the ray enters.
+template <int NB, int NF>
+__device__ __forceinline__ uint32_t group_mask(const float* __restrict__ g, const Ray32& r) {
+    float v[NF];
+    const float4* g4 = reinterpret_cast<const float4*>(g);
+#pragma unroll
+    for (int q = 0; q < NF / 4; ++q) {
+        const float4 x = g4[q];
+        v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+        if (box32(v + 6 * q, r)) m |= 1u << q;
+    return m;
+}
+
+// Stage 1 through the culling pyramid (grouped layout: one load per level).
 __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, const double* C, const V3& o,
                               const V3& d, const Ray32& r, Stage1& best, Counters& c) {
-    const float* bs = S.box_super + (size_t)k * bbrt::kSuperN * bbrt::kSuperN * 6;
-    const float* bb = S.box_block + (size_t)k * bbrt::kBlockN * bbrt::kBlockN * 6;
-    const float* bt = S.box_tile + (size_t)k * bbrt::kTileN * bbrt::kTileN * 6;
-    const float* bc = S.box_cell + (size_t)k * bbrt::kCells * bbrt::kCells * 6;
-    for (int si = 0; si < bbrt::kSuperN; ++si)
-        for (int sj = 0; sj < bbrt::kSuperN; ++sj) {
-            ++c.cull_tests;
-            if (!box32(bs + (si * bbrt::kSuperN + sj) * 6, r)) continue;
-            const int bi_end = min(bbrt::kBlockN, bbrt::kBlocksPerSuper * (si + 1));
-            const int bj_end = min(bbrt::kBlockN, bbrt::kBlocksPerSuper * (sj + 1));
-            for (int bi = bbrt::kBlocksPerSuper * si; bi < bi_end; ++bi)
-                for (int bj = bbrt::kBlocksPerSuper * sj; bj < bj_end; ++bj) {
-                    ++c.cull_tests;
-                    if (!box32(bb + (bi * bbrt::kBlockN + bj) * 6, r)) continue;
-                    for (int ti = 3 * bi; ti < 3 * bi + 3; ++ti)
-                        for (int tj = 3 * bj; tj < 3 * bj + 3; ++tj) {
-                            ++c.cull_tests;
-                            if (!box32(bt + (ti * bbrt::kTileN + tj) * 6, r)) continue;
-                            for (int i = 3 * ti; i < 3 * ti + 3; ++i)
-                                for (int j = 3 * tj; j < 3 * tj + 3; ++j) {
-                                    ++c.cull_tests;
-                                    if (!box32(bc + (i * bbrt::kCells + j) * 6, r)) continue;
-                                    cell_tests(G, C, i, j, o, d, best, c);
-                                }
-                        }
+    const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
+    const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
+    const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
+    const float* gc = S.box_cell + (size_t)k * bbrt::kCellFloats;
+    uint32_t ms = group_mask<9, bbrt::kG9>(gs, r);
+    c.cull_tests += 9;
+    while (ms) {
+        const int sq = __ffs(ms) - 1;
+        ms &= ms - 1;
+        const int si = sq / 3, sj = sq % 3;
+        uint32_t mb = group_mask<16, bbrt::kG16>(gb + sq * bbrt::kG16, r);
+        c.cull_tests += 16;
+        while (mb) {
+            const int bq = __ffs(mb) - 1;
+            mb &= mb - 1;
+            const int bi = bbrt::kBlocksPerSuper * si + (bq >> 2), bj = bbrt::kBlocksPerSuper * sj + (bq & 3);
+            uint32_t mt = group_mask<9, bbrt::kG9>(gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9, r);
+            c.cull_tests += 9;
+            while (mt) {
+                const int tq = __ffs(mt) - 1;
+                mt &= mt - 1;
+                const int ti = 3 * bi + tq / 3, tj = 3 * bj + tq % 3;
+                uint32_t mc = group_mask<9, bbrt::kG9>(gc + (ti * bbrt::kTileN + tj) * bbrt::kG9, r);
+                c.cull_tests += 9;
+                while (mc) {
+                    const int cq = __ffs(mc) - 1;
+                    mc &= mc - 1;
+                    cell_tests(G, C, 3 * ti + cq / 3, 3 * tj + cq % 3, o, d, best, c);
                 }
+            }
         }
+    }
 }
 
 struct PHit {
@@ -453,23 +478,177 @@ __device__ __forceinline__ void write_final(const Outs& O, int32_t r, const V3& 
     }
 }
 
-// RayTracingDriver::trace_ray, lines 172-184: zero-direction check, normalize_dir
-// (:398-401), per-ray output initialisation.
-__global__ void k_init(const double* __restrict__ origins, const double* __restrict__ dirs, int64_t n, RayState R,
-                       Outs O, int record_k, int max_bounces, int32_t* live_out, int32_t* n_out,
-                       unsigned long long* ctr) {
+// The part of ray_box (BVH.hpp:14-37) that does not depend on best_t: false
+// when the box is rejected whatever best_t is, else the entry distance.
+__device__ __forceinline__ bool box_static(const V3& o, const double inv[3], const BvhNode& nd, double& tmin_out) {
+    double tmin = -INFINITY, tmax = INFINITY;
+    const double oo[3] = { o.x, o.y, o.z };
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double t1 = (nd.lo[a] - oo[a]) * inv[a];
+        double t2 = (nd.hi[a] - oo[a]) * inv[a];
+        if (inv[a] < 0.0) { const double s = t1; t1 = t2; t2 = s; }
+        tmin = smax(tmin, t1);
+        tmax = smin(tmax, t2);
+        if (tmax < tmin) return false;
+    }
+    tmin_out = tmin;
+    return tmax >= smax(tmin, 0.0);
+}
+
+// Traversal stack entry: idx < 0 marks a node whose box test already failed;
+// meta >= 0 is an internal node (right child << 2 | split axis), meta < 0 a
+// leaf (-1 - (start << 3 | count)); tmin is the node's entry distance.
+struct Ent {
+    int32_t idx;
+    int32_t meta;
+    double tmin;
+};
+
+__device__ __forceinline__ Ent make_ent(int32_t idx, const BvhNode& nd, bool ok, double tmin) {
+    Ent e;
+    e.idx = ok ? idx : -1;
+    e.meta = (nd.right >= 0) ? ((nd.right << 2) | nd.axis) : (-1 - ((nd.start << 3) | nd.count));
+    e.tmin = tmin;
+    return e;
+}
+
+// BVH::intersect (BVH.hpp:75-128).  A node's box is tested when its parent is
+// expanded: the best_t-independent part of ray_box then, the `tmin > best_t`
+// prune when the node is popped -- the same outcome as testing it whole at the
+// pop, since best_t only decreases.  A child that fails is still pushed (as a
+// dead entry) so the visit order, the stack depth and the node-test count are
+// the reference's.  The near child stays in registers instead of being pushed
+// and popped at once.  Returns 0 or an error kind.
+__device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32& r32, bool cull, int mode,
+                        bool root_live, PHit& best, int32_t& best_obj, int32_t& err_obj, Counters& c) {
+    const double inv[3] = { 1.0 / d.x, 1.0 / d.y, 1.0 / d.z };
+    const bool neg[3] = { inv[0] < 0.0, inv[1] < 0.0, inv[2] < 0.0 };
+    double best_t = INFINITY;
+    best_obj = -1;
+    err_obj = -1;
+    Ent stack[kStackCap];
+    int sp = 0;
+    Ent cur;
+    {
+        const BvhNode root = S.bvh[0];
+        double tm = -INFINITY;
+        const bool ok = root_live ? true : box_static(o, inv, root, tm);
+        cur = make_ent(0, root, ok, tm);
+    }
+    for (;;) {
+        c.bvh_tests++;
+        const Ent e = cur;
+        bool expand = e.idx >= 0 && !(e.tmin > best_t);
+        if (expand && e.meta < 0) {
+            const int v = -1 - e.meta;
+            const int start = v >> 3, count = v & 7;
+            for (int q = 0; q < count; ++q) {
+                const int32_t obj = S.leaf[start + q];
+                const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
+                if (h.status < 0) { err_obj = obj; return -h.status; }
+                if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
+            }
+            expand = false;
+        }
+        if (!expand) {
+            if (sp == 0) break;
+            cur = stack[--sp];
+            continue;
+        }
+        const int32_t left = e.idx + 1, right = e.meta >> 2;
+        const BvhNode nl = S.bvh[left];
+        const BvhNode nr = S.bvh[right];
+        double tl = 0.0, tr = 0.0;
+        const bool okl = box_static(o, inv, nl, tl);
+        const bool okr = box_static(o, inv, nr, tr);
+        if (sp + 1 > kStackCap) return bbrt::kErrStack;
+        const Ent el = make_ent(left, nl, okl, tl), er = make_ent(right, nr, okr, tr);
+        if (neg[e.meta & 3]) { stack[sp++] = el; cur = er; }
+        else { stack[sp++] = er; cur = el; }
+    }
+    return 0;
+}
+
+// One bounce of ray r: RayTracingDriver.hpp:186-237.  Returns true while the
+// ray continues; o and d are advanced in place.
+__device__ bool bounce_once(const DevScene& S, const Outs& O, int32_t r, V3& o, V3& d, int bounce, int max_bounces,
+                            int record_k, int mode, uint32_t flags, bool root_live, Counters& c) {
+    const Counters c0 = c;
+    const bool finite = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(d.x) && isfinite(d.y) &&
+                        isfinite(d.z);
+    const bool cull = !(flags & BBRT_FLAG_NO_CULL) && finite && fabs(o.x) <= S.cull_limit &&
+                      fabs(o.y) <= S.cull_limit && fabs(o.z) <= S.cull_limit;
+    const Ray32 r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
+    PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+    int32_t best_obj, err_obj;
+    const int err = traverse(S, o, d, r32, cull, mode, root_live, best, best_obj, err_obj, c);
+    bool alive = false;
+    if (err) {
+        if (O.status) O.status[r] = bbrt::kError;
+        if (O.err_kind) O.err_kind[r] = (int8_t)err;
+        if (O.err_patch) O.err_patch[r] = err_obj;
+        if (O.bounces) O.bounces[r] = bounce;
+        write_final(O, r, o, d);
+    } else if (best_obj < 0) {
+        if (O.status) O.status[r] = bbrt::kEscaped;
+        if (O.bounces) O.bounces[r] = bounce;
+        write_final(O, r, o, d);
+    } else if (dot3(d, best.n) > kBackfaceTol) {
+        if (O.status) O.status[r] = bbrt::kError;
+        if (O.err_kind) O.err_kind[r] = bbrt::kErrBackface;
+        if (O.err_patch) O.err_patch[r] = best_obj;
+        if (O.bounces) O.bounces[r] = bounce;
+        write_final(O, r, o, d);
+    } else {
+        if (bounce < record_k) {
+            const size_t at = (size_t)r * record_k + bounce;
+            if (O.hit_patch) O.hit_patch[at] = best_obj;
+            if (O.newton_iters) O.newton_iters[at] = (uint8_t)best.iters;
+            if (O.hit_xyz) { O.hit_xyz[3 * at] = best.p.x; O.hit_xyz[3 * at + 1] = best.p.y; O.hit_xyz[3 * at + 2] = best.p.z; }
+        }
+        if (O.hit_count && (bounce >= 1 || (flags & BBRT_FLAG_COUNT_DIRECT_HIT)))
+            atomicAdd(O.hit_count + best_obj, 1ull);
+        c.bounces++;
+        // reflect (RayTracingDriver.hpp:405-408) and nudge (:234-237, :260-265)
+        const V3& nn = best.n;
+        const double dn = d.x * nn.x + d.y * nn.y + d.z * nn.z;
+        d = { d.x - 2.0 * dn * nn.x, d.y - 2.0 * dn * nn.y, d.z - 2.0 * dn * nn.z };
+        const double eps = smax(kSelfEpsFactor * S.diag, kSagittaClearance * S.meta[best_obj].max_sag);
+        o = { best.p.x + d.x * eps, best.p.y + d.y * eps, best.p.z + d.z * eps };
+        if (bounce + 1 >= max_bounces) {
+            if (O.status) O.status[r] = bbrt::kTrapped;
+            if (O.bounces) O.bounces[r] = bounce + 1;
+            write_final(O, r, o, d);
+        } else {
+            alive = true;
+        }
+    }
+    if (O.ray_work) {
+        uint32_t* w = O.ray_work + 4 * (size_t)r;
+        w[0] += (uint32_t)(c.bvh_tests - c0.bvh_tests);
+        w[1] += (uint32_t)(c.cull_tests - c0.cull_tests);
+        w[2] += (uint32_t)(c.mt_tests - c0.mt_tests);
+        w[3] += (uint32_t)(c.newton_iters - c0.newton_iters);
+    }
+    return alive;
+}
+
+// RayTracingDriver::trace_ray, lines 172-184: zero-direction check and
+// normalize_dir (:398-401); then the root box test of the first BVH query
+// (BVH.hpp:91-97 with best_t = inf), so rays that miss the whole scene leave
+// here as Escaped after one node test.  The rest enter the live list.
+__global__ void k_init(const double* __restrict__ origins, const double* __restrict__ dirs, int64_t n,
+                       const BvhNode* __restrict__ bvh, RayState R, Outs O, int max_bounces, int32_t* live_out,
+                       int32_t* n_out, unsigned long long* ctr) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
     int32_t r = (int32_t)g;
+    unsigned long long root_tests = 0;
     if (g < n) {
         const V3 o = { origins[3 * g], origins[3 * g + 1], origins[3 * g + 2] };
         const V3 dr = { dirs[3 * g], dirs[3 * g + 1], dirs[3 * g + 2] };
-        for (int q = 0; q < record_k; ++q) {
-            const size_t at = (size_t)g * record_k + q;
-            if (O.hit_patch) O.hit_patch[at] = -1;
-            if (O.newton_iters) O.newton_iters[at] = 0;
-            if (O.hit_xyz) { O.hit_xyz[3 * at] = 0.0; O.hit_xyz[3 * at + 1] = 0.0; O.hit_xyz[3 * at + 2] = 0.0; }
-        }
+        if (O.ray_work) { O.ray_work[4 * g] = 0; O.ray_work[4 * g + 1] = 0; O.ray_work[4 * g + 2] = 0; O.ray_work[4 * g + 3] = 0; }
         if (O.err_kind) O.err_kind[g] = bbrt::kErrNone;
         if (O.err_patch) O.err_patch[g] = -1;
         const double len2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
@@ -481,9 +660,17 @@ __global__ void k_init(const double* __restrict__ origins, const double* __restr
         } else {
             const double len = sqrt(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
             const V3 d = { dr.x / len, dr.y / len, dr.z / len };
+            const double inv[3] = { 1.0 / d.x, 1.0 / d.y, 1.0 / d.z };
+            double tm;
             if (max_bounces <= 0) {
                 if (O.status) O.status[g] = bbrt::kTrapped;
                 if (O.bounces) O.bounces[g] = 0;
+                write_final(O, r, o, d);
+            } else if (!box_static(o, inv, bvh[0], tm)) {
+                root_tests = 1;
+                if (O.status) O.status[g] = bbrt::kEscaped;
+                if (O.bounces) O.bounces[g] = 0;
+                if (O.ray_work) O.ray_work[4 * g] = 1;
                 write_final(O, r, o, d);
             } else {
                 R.o[3 * g] = o.x; R.o[3 * g + 1] = o.y; R.o[3 * g + 2] = o.z;
@@ -494,10 +681,58 @@ __global__ void k_init(const double* __restrict__ origins, const double* __restr
     }
     append_live(alive, r, live_out, n_out);
     const unsigned long long in_range = __ballot(g < n);
-    if (__lane_id() == 0 && in_range) atomicAdd(ctr + bbrt::kCtrRays, (unsigned long long)__popcll(in_range));
+    const unsigned long long rt = wave_sum(root_tests);
+    if (__lane_id() == 0) {
+        if (in_range) atomicAdd(ctr + bbrt::kCtrRays, (unsigned long long)__popcll(in_range));
+        if (rt) atomicAdd(ctr + bbrt::kCtrBvh, rt);
+    }
 }
 
-// One bounce of every live ray: RayTracingDriver.hpp:186-237.
+// Persistent bounce loop: lanes take rays from the live list through one
+// wave-aggregated atomic, run them bounce after bounce with the ray state in
+// registers, and take the next ray when one ends.  Every wave leaves once the
+// list is exhausted and its lanes' rays have ended (each ray ends within
+// max_bounces bounces).
+template <int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k_persist(DevScene S, RayState R, const int32_t* __restrict__ live,
+                                                 const int32_t* __restrict__ n_live_p, int32_t* head, Outs O,
+                                                 int max_bounces, int record_k, int mode, uint32_t flags,
+                                                 unsigned long long* ctr) {
+    const int n_live = *n_live_p;
+    const int lane = __lane_id();
+    Counters c = {};
+    int32_t r = -1;   // -1: wants a ray, -2: list exhausted
+    int b = 0;
+    V3 o = { 0, 0, 0 }, d = { 0, 0, 0 };
+    for (;;) {
+        const unsigned long long want = __ballot(r == -1);
+        if (want) {
+            const int leader = __ffsll((long long)want) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(head, (int)__popcll(want));
+            base = __shfl(base, leader);
+            if (r == -1) {
+                const int q = base + (int)__popcll(want & ((1ull << lane) - 1ull));
+                if (q < n_live) {
+                    r = live[q];
+                    o = { R.o[3 * r], R.o[3 * r + 1], R.o[3 * r + 2] };
+                    d = { R.d[3 * r], R.d[3 * r + 1], R.d[3 * r + 2] };
+                    b = 0;
+                } else {
+                    r = -2;
+                }
+            }
+        }
+        if (__ballot(r >= 0) == 0) break;
+        if (r >= 0) {
+            if (bounce_once(S, O, r, o, d, b, max_bounces, record_k, mode, flags, b == 0, c)) ++b;
+            else r = -1;
+        }
+    }
+    flush_counters(c, ctr);
+}
+
+// One bounce of every live ray, compacting the survivors (the wavefront schedule).
 __global__ void __launch_bounds__(256) k_bounce(DevScene S, RayState R, const int32_t* __restrict__ live_in,
                                                 const int32_t* __restrict__ n_in, int32_t* live_out, int32_t* n_out,
                                                 Outs O, int bounce, int max_bounces, int record_k, int mode,
@@ -511,87 +746,23 @@ __global__ void __launch_bounds__(256) k_bounce(DevScene S, RayState R, const in
         r = live_in[g];
         V3 o = { R.o[3 * r], R.o[3 * r + 1], R.o[3 * r + 2] };
         V3 d = { R.d[3 * r], R.d[3 * r + 1], R.d[3 * r + 2] };
-        const double inv[3] = { 1.0 / d.x, 1.0 / d.y, 1.0 / d.z };
-        const bool neg[3] = { inv[0] < 0.0, inv[1] < 0.0, inv[2] < 0.0 };
-        const bool finite = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(d.x) && isfinite(d.y) &&
-                            isfinite(d.z);
-        const bool cull = !(flags & BBRT_FLAG_NO_CULL) && finite && fabs(o.x) <= S.cull_limit &&
-                          fabs(o.y) <= S.cull_limit && fabs(o.z) <= S.cull_limit;
-        const Ray32 r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y,
-                            1.0f / (float)d.z };
-        // BVH::intersect (BVH.hpp:75-128)
-        PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
-        double best_t = INFINITY;
-        int32_t best_obj = -1, err_obj = -1;
-        int err = 0;
-        int32_t stack[kStackCap];
-        int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            const int32_t idx = stack[--sp];
-            const BvhNode nd = S.bvh[idx];
-            c.bvh_tests++;
-            if (!ray_box64(o, inv, nd, best_t)) continue;
-            if (nd.right < 0) {
-                for (int32_t q = 0; q < nd.count; ++q) {
-                    const int32_t obj = S.leaf[nd.start + q];
-                    const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
-                    if (h.status < 0) { err = -h.status; err_obj = obj; break; }
-                    if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
-                }
-                if (err) break;
-                continue;
-            }
-            if (sp + 2 > kStackCap) { err = bbrt::kErrStack; break; }
-            const int32_t left = idx + 1, right = nd.right;
-            if (neg[nd.axis]) { stack[sp++] = left; stack[sp++] = right; }
-            else { stack[sp++] = right; stack[sp++] = left; }
-        }
-        if (err) {
-            if (O.status) O.status[r] = bbrt::kError;
-            if (O.err_kind) O.err_kind[r] = (int8_t)err;
-            if (O.err_patch) O.err_patch[r] = err_obj;
-            if (O.bounces) O.bounces[r] = bounce;
-            write_final(O, r, o, d);
-        } else if (best_obj < 0) {
-            if (O.status) O.status[r] = bbrt::kEscaped;
-            if (O.bounces) O.bounces[r] = bounce;
-            write_final(O, r, o, d);
-        } else if (dot3(d, best.n) > kBackfaceTol) {
-            if (O.status) O.status[r] = bbrt::kError;
-            if (O.err_kind) O.err_kind[r] = bbrt::kErrBackface;
-            if (O.err_patch) O.err_patch[r] = best_obj;
-            if (O.bounces) O.bounces[r] = bounce;
-            write_final(O, r, o, d);
-        } else {
-            if (bounce < record_k) {
-                const size_t at = (size_t)r * record_k + bounce;
-                if (O.hit_patch) O.hit_patch[at] = best_obj;
-                if (O.newton_iters) O.newton_iters[at] = (uint8_t)best.iters;
-                if (O.hit_xyz) { O.hit_xyz[3 * at] = best.p.x; O.hit_xyz[3 * at + 1] = best.p.y; O.hit_xyz[3 * at + 2] = best.p.z; }
-            }
-            if (O.hit_count && (bounce >= 1 || (flags & BBRT_FLAG_COUNT_DIRECT_HIT)))
-                atomicAdd(O.hit_count + best_obj, 1ull);
-            c.bounces++;
-            // reflect (RayTracingDriver.hpp:405-408) and nudge (:234-237, :260-265)
-            const V3& nn = best.n;
-            const double dn = d.x * nn.x + d.y * nn.y + d.z * nn.z;
-            d = { d.x - 2.0 * dn * nn.x, d.y - 2.0 * dn * nn.y, d.z - 2.0 * dn * nn.z };
-            const double eps = smax(kSelfEpsFactor * S.diag, kSagittaClearance * S.meta[best_obj].max_sag);
-            o = { best.p.x + d.x * eps, best.p.y + d.y * eps, best.p.z + d.z * eps };
-            if (bounce + 1 >= max_bounces) {
-                if (O.status) O.status[r] = bbrt::kTrapped;
-                if (O.bounces) O.bounces[r] = bounce + 1;
-                write_final(O, r, o, d);
-            } else {
-                R.o[3 * r] = o.x; R.o[3 * r + 1] = o.y; R.o[3 * r + 2] = o.z;
-                R.d[3 * r] = d.x; R.d[3 * r + 1] = d.y; R.d[3 * r + 2] = d.z;
-                alive = true;
-            }
+        alive = bounce_once(S, O, r, o, d, bounce, max_bounces, record_k, mode, flags, bounce == 0, c);
+        if (alive) {
+            R.o[3 * r] = o.x; R.o[3 * r + 1] = o.y; R.o[3 * r + 2] = o.z;
+            R.d[3 * r] = d.x; R.d[3 * r + 1] = d.y; R.d[3 * r + 2] = d.z;
         }
     }
     append_live(alive, r, live_out, n_out);
     flush_counters(c, ctr);
+}
+
+// -1 / 0 padding of the per-bounce records (coalesced, grid-stride).
+__global__ void k_pad_records(Outs O, int64_t total) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        if (O.hit_patch) O.hit_patch[q] = -1;
+        if (O.newton_iters) O.newton_iters[q] = 0;
+        if (O.hit_xyz) { O.hit_xyz[3 * q] = 0.0; O.hit_xyz[3 * q + 1] = 0.0; O.hit_xyz[3 * q + 2] = 0.0; }
+    }
 }
 
 __global__ void k_zero_i32(int32_t* p, int n) {
@@ -625,6 +796,7 @@ T* dev_upload(const std::vector<T>& v, std::vector<void*>& allocs, int64_t& byte
 
 struct bbrt_scene_s {
     int device = 0;
+    int n_cu = 1;
     hipStream_t stream = nullptr;
     DevScene dev{};
     std::vector<void*> allocs;
@@ -673,7 +845,7 @@ void* scratch_for(bbrt_scene s, size_t need) {
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, const bbrt_params* p,
-                const bbrt_outputs* out, int64_t* counters, hipStream_t st, double* kernel_ms) {
+                const bbrt_outputs* out, int64_t* counters, hipStream_t st, double* kernel_ms, bool pad_records) {
     if (!p) throw std::runtime_error("bbrt_trace: params is NULL");
     if (p->max_bounces < 0 || p->record_k < 0) throw std::runtime_error("bbrt_trace: negative max_bounces or record_k");
     if (p->newton_mode != BBRT_NEWTON_REF && p->newton_mode != BBRT_NEWTON_CONVERGED)
@@ -702,22 +874,14 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     char* base = static_cast<char*>(scratch_for(s, need));
     RayState R{ reinterpret_cast<double*>(base + off_o), reinterpret_cast<double*>(base + off_d) };
     int32_t* live[2] = { reinterpret_cast<int32_t*>(base + off_la), reinterpret_cast<int32_t*>(base + off_lb) };
-    int32_t* cnt = reinterpret_cast<int32_t*>(base + off_cnt);   // cnt[0], cnt[1]
+    int32_t* cnt = reinterpret_cast<int32_t*>(base + off_cnt);   // [0], [1]: list sizes, [2]: queue head
     unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base + off_ctr);
     Outs O{ out ? out->status : nullptr, out ? out->err_kind : nullptr, out ? out->bounces : nullptr,
             out ? out->err_patch : nullptr, out ? out->hit_patch : nullptr, out ? out->hit_xyz : nullptr,
             out ? out->newton_iters : nullptr, out ? out->final_od : nullptr,
-            out ? reinterpret_cast<unsigned long long*>(out->hit_count) : nullptr };
+            out ? reinterpret_cast<unsigned long long*>(out->hit_count) : nullptr, out ? out->ray_work : nullptr };
     const bool timing = (p->flags & BBRT_FLAG_TIME_KERNELS) != 0;
-    check(hipEventRecord(s->ev[0], st), "hipEventRecord");
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cnt, 2);
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t*>(ctr), 2 * BBRT_NCOUNTERS);
-    const int64_t grid0 = (n + 255) / 256;
-    hipLaunchKernelGGL(k_init, dim3((unsigned)grid0), dim3(256), 0, st, d_o, d_d, n, R, O, p->record_k,
-                       p->max_bounces, live[0], cnt, ctr);
-    check(hipGetLastError(), "k_init launch");
-    int cur = 0;
-    int launches = 0;
+    const bool wavefront = (p->flags & BBRT_FLAG_WAVEFRONT) != 0;
     if (timing && (int)s->kev.size() < 2 * p->max_bounces + 2) {
         const size_t want = 2 * (size_t)p->max_bounces + 2;
         while (s->kev.size() < want) {
@@ -726,21 +890,58 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             s->kev.push_back(e);
         }
     }
-    for (int b = 0; b < p->max_bounces; ++b) {
-        check(hipMemcpyAsync(s->h_count, cnt + cur, sizeof(int32_t), hipMemcpyDeviceToHost, st), "count D2H");
-        check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        const int32_t n_live = *s->h_count;
-        if (n_live <= 0) break;
-        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cnt + (cur ^ 1), 1);
-        const unsigned grid = (unsigned)((n_live + block - 1) / block);
-        if (timing) check(hipEventRecord(s->kev[2 * launches], st), "hipEventRecord");
-        hipLaunchKernelGGL(k_bounce, dim3(grid), dim3(block), 0, st, s->dev, R, live[cur], cnt + cur,
-                           live[cur ^ 1], cnt + (cur ^ 1), O, b, p->max_bounces, p->record_k, p->newton_mode,
-                           p->flags, ctr);
-        if (timing) check(hipEventRecord(s->kev[2 * launches + 1], st), "hipEventRecord");
-        check(hipGetLastError(), "k_bounce launch");
-        ++launches;
-        cur ^= 1;
+    check(hipEventRecord(s->ev[0], st), "hipEventRecord");
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cnt, 4);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t*>(ctr), 2 * BBRT_NCOUNTERS);
+    if (pad_records && p->record_k > 0) {
+        const int64_t total = n * (int64_t)p->record_k;
+        const unsigned pg = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_pad_records, dim3(pg), dim3(256), 0, st, O, total);
+    }
+    const int64_t grid0 = (n + 255) / 256;
+    hipLaunchKernelGGL(k_init, dim3((unsigned)grid0), dim3(256), 0, st, d_o, d_d, n, s->dev.bvh, R, O,
+                       p->max_bounces, live[0], cnt, ctr);
+    check(hipGetLastError(), "k_init launch");
+    int launches = 0;
+    if (!wavefront) {
+        if (p->max_bounces > 0) {
+            const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : kDefaultWavesPerEu;
+            decltype(&k_persist<1>) kern = nullptr;
+            switch (wpe) {
+                case 1: case 2: kern = &k_persist<1>; break;
+                case 3: kern = &k_persist<3>; break;
+                case 4: kern = &k_persist<4>; break;
+                default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
+            }
+            int per_cu = 0;
+            check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0), "occupancy");
+            const int64_t resident = (int64_t)std::max(1, per_cu) * s->n_cu;
+            const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, resident));
+            if (timing) check(hipEventRecord(s->kev[0], st), "hipEventRecord");
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s->dev, R, live[0], cnt, cnt + 2, O,
+                               p->max_bounces, p->record_k, p->newton_mode, p->flags, ctr);
+            if (timing) check(hipEventRecord(s->kev[1], st), "hipEventRecord");
+            check(hipGetLastError(), "k_persist launch");
+            launches = 1;
+        }
+    } else {
+        int cur = 0;
+        for (int b = 0; b < p->max_bounces; ++b) {
+            check(hipMemcpyAsync(s->h_count, cnt + cur, sizeof(int32_t), hipMemcpyDeviceToHost, st), "count D2H");
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            const int32_t n_live = *s->h_count;
+            if (n_live <= 0) break;
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cnt + (cur ^ 1), 1);
+            const unsigned grid = (unsigned)((n_live + block - 1) / block);
+            if (timing) check(hipEventRecord(s->kev[2 * launches], st), "hipEventRecord");
+            hipLaunchKernelGGL(k_bounce, dim3(grid), dim3(block), 0, st, s->dev, R, live[cur], cnt + cur,
+                               live[cur ^ 1], cnt + (cur ^ 1), O, b, p->max_bounces, p->record_k, p->newton_mode,
+                               p->flags, ctr);
+            if (timing) check(hipEventRecord(s->kev[2 * launches + 1], st), "hipEventRecord");
+            check(hipGetLastError(), "k_bounce launch");
+            ++launches;
+            cur ^= 1;
+        }
     }
     check(hipEventRecord(s->ev[1], st), "hipEventRecord");
     unsigned long long hc[BBRT_NCOUNTERS];
@@ -828,6 +1029,10 @@ int bbrt_scene_create(const bbrt_patch_set* ps, bbrt_scene* out) {
         bbrt::build_host_scene(*ps, H, (int)std::min(16u, std::max(1u, hw)));
         s = new bbrt_scene_s();
         check(hipGetDevice(&s->device), "hipGetDevice");
+        check(hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, s->device), "CU count");
+        for (const BvhNode& nd : H.bvh)
+            if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
+                throw std::runtime_error("bbrt_scene_create: BVH exceeds the packed node limits");
         check(hipStreamCreate(&s->stream), "hipStreamCreate");
         check(hipEventCreate(&s->ev[0]), "hipEventCreate");
         check(hipEventCreate(&s->ev[1]), "hipEventCreate");
@@ -927,7 +1132,7 @@ int bbrt_trace_device(bbrt_scene s, const double* origins, const double* dirs, i
                       const bbrt_outputs* out, int64_t* counters, void* stream, double* kernel_ms) {
     if (!s) { g_last_error = "bbrt_trace_device: NULL scene"; return -1; }
     try {
-        trace_impl(s, origins, dirs, n, p, out, counters, static_cast<hipStream_t>(stream), kernel_ms);
+        trace_impl(s, origins, dirs, n, p, out, counters, static_cast<hipStream_t>(stream), kernel_ms, false);
         return 0;
     } catch (const std::exception& e) {
         return fail(e);
@@ -967,11 +1172,12 @@ int bbrt_trace(bbrt_scene s, const double* origins, const double* dirs, int64_t 
         if (hout.hit_xyz) dout.hit_xyz = static_cast<double*>(dalloc(nn * K * 24));
         if (hout.newton_iters) dout.newton_iters = static_cast<uint8_t*>(dalloc(nn * K));
         if (hout.final_od) dout.final_od = static_cast<double*>(dalloc(nn * 48));
+        if (hout.ray_work) dout.ray_work = static_cast<uint32_t*>(dalloc(nn * 16));
         if (hout.hit_count) {
             dout.hit_count = static_cast<int64_t*>(dalloc((size_t)s->n_patches * 8));
             check(hipMemcpy(dout.hit_count, hout.hit_count, (size_t)s->n_patches * 8, hipMemcpyHostToDevice), "H2D hit_count");
         }
-        trace_impl(s, d_o, d_d, n, p, &dout, counters, s->stream, nullptr);
+        trace_impl(s, d_o, d_d, n, p, &dout, counters, s->stream, nullptr, true);
         auto back = [&](void* h, const void* d, size_t bytes) {
             if (h && bytes) check(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost), "D2H outputs");
         };
@@ -984,6 +1190,7 @@ int bbrt_trace(bbrt_scene s, const double* origins, const double* dirs, int64_t 
         back(hout.newton_iters, dout.newton_iters, nn * K);
         back(hout.final_od, dout.final_od, nn * 48);
         back(hout.hit_count, dout.hit_count, (size_t)s->n_patches * 8);
+        back(hout.ray_work, dout.ray_work, nn * 16);
         cleanup();
         return 0;
     } catch (const std::exception& e) {

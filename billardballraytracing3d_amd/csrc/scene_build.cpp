@@ -218,27 +218,34 @@ void prepare_patch(PatchWork& W, int64_t k) {
     W.hmax[k] = h;
 }
 
-// FP32 culling pyramid of one patch, widened by `widen`.
+// FP32 culling pyramid of one patch, widened by `widen`, in the grouped layout
+// described in scene_host.h.
 void build_pyramid(HostScene& S, int64_t k, double widen) {
     const double* G = &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
     const double* C = &S.cell[(size_t)k * kCells * kCells * 4];
-    float* bc = &S.box_cell[(size_t)k * kCells * kCells * 6];
-    float* bt = &S.box_tile[(size_t)k * kTileN * kTileN * 6];
-    float* bb = &S.box_block[(size_t)k * kBlockN * kBlockN * 6];
-    float* bs = &S.box_super[(size_t)k * kSuperN * kSuperN * 6];
+    float* gc = &S.box_cell[(size_t)k * kCellFloats];
+    float* gt = &S.box_tile[(size_t)k * kTileFloats];
+    float* gb = &S.box_block[(size_t)k * kBlockFloats];
+    float* gs = &S.box_super[(size_t)k * kSuperFloats];
     const float inf = std::numeric_limits<float>::infinity();
     auto empty = [&](float* b) { b[0] = b[1] = b[2] = inf; b[3] = b[4] = b[5] = -inf; };
     auto merge = [&](float* dst, const float* src) {
         for (int a = 0; a < 3; ++a) { dst[a] = std::min(dst[a], src[a]); dst[3 + a] = std::max(dst[3 + a], src[3 + a]); }
     };
-    if (!S.meta[k].cull) {
-        // Never consulted (the kernels test such a patch exhaustively); keep it finite.
-        for (size_t q = 0; q < (size_t)kCells * kCells; ++q) empty(bc + 6 * q);
-        for (size_t q = 0; q < (size_t)kTileN * kTileN; ++q) empty(bt + 6 * q);
-        for (size_t q = 0; q < (size_t)kBlockN * kBlockN; ++q) empty(bb + 6 * q);
-        for (size_t q = 0; q < (size_t)kSuperN * kSuperN; ++q) empty(bs + 6 * q);
-        return;
-    }
+    // every box slot starts empty, padding floats zero
+    for (size_t q = 0; q < kTileFloats; ++q) gt[q] = 0.0f;
+    for (size_t q = 0; q < kBlockFloats; ++q) gb[q] = 0.0f;
+    for (size_t q = 0; q < kSuperFloats; ++q) gs[q] = 0.0f;
+    for (size_t q = 0; q < kCellFloats; ++q) gc[q] = 0.0f;
+    auto empty_group = [&](float* g, int nbox) { for (int q = 0; q < nbox; ++q) empty(g + 6 * q); };
+    for (int t = 0; t < kTileN * kTileN; ++t) empty_group(gc + (size_t)t * kG9, 9);
+    for (int b = 0; b < kBlockN * kBlockN; ++b) empty_group(gt + (size_t)b * kG9, 9);
+    for (int s = 0; s < kSuperN * kSuperN; ++s) empty_group(gb + (size_t)s * kG16, 16);
+    empty_group(gs, 9);
+    if (!S.meta[k].cull) return;   // never consulted: such a patch is tested exhaustively
+
+    std::vector<float> cellb((size_t)kCells * kCells * 6), tileb((size_t)kTileN * kTileN * 6),
+        blockb((size_t)kBlockN * kBlockN * 6);
     for (int i = 0; i < kCells; ++i)
         for (int j = 0; j < kCells; ++j) {
             const double* c = &C[((size_t)i * kCells + j) * 4];
@@ -257,30 +264,38 @@ void build_pyramid(HostScene& S, int64_t k, double widen) {
                     for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], q[a]); hi[a] = std::max(hi[a], q[a]); }
                 }
             }
-            float* b = bc + ((size_t)i * kCells + j) * 6;
+            float* b = &cellb[((size_t)i * kCells + j) * 6];
             for (int a = 0; a < 3; ++a) { b[a] = round_down(lo[a] - widen); b[3 + a] = round_up(hi[a] + widen); }
+            // grouped: tile (i/3, j/3), slot 3*(i%3) + j%3
+            float* g = gc + (size_t)((i / 3) * kTileN + j / 3) * kG9 + 6 * (3 * (i % 3) + j % 3);
+            for (int a = 0; a < 6; ++a) g[a] = b[a];
         }
     for (int ti = 0; ti < kTileN; ++ti)
         for (int tj = 0; tj < kTileN; ++tj) {
-            float* b = bt + ((size_t)ti * kTileN + tj) * 6;
+            float* b = &tileb[((size_t)ti * kTileN + tj) * 6];
             empty(b);
             for (int i = 3 * ti; i < 3 * ti + 3; ++i)
-                for (int j = 3 * tj; j < 3 * tj + 3; ++j) merge(b, bc + ((size_t)i * kCells + j) * 6);
+                for (int j = 3 * tj; j < 3 * tj + 3; ++j) merge(b, &cellb[((size_t)i * kCells + j) * 6]);
+            float* g = gt + (size_t)((ti / 3) * kBlockN + tj / 3) * kG9 + 6 * (3 * (ti % 3) + tj % 3);
+            for (int a = 0; a < 6; ++a) g[a] = b[a];
         }
     for (int bi = 0; bi < kBlockN; ++bi)
         for (int bj = 0; bj < kBlockN; ++bj) {
-            float* b = bb + ((size_t)bi * kBlockN + bj) * 6;
+            float* b = &blockb[((size_t)bi * kBlockN + bj) * 6];
             empty(b);
             for (int ti = 3 * bi; ti < 3 * bi + 3; ++ti)
-                for (int tj = 3 * bj; tj < 3 * bj + 3; ++tj) merge(b, bt + ((size_t)ti * kTileN + tj) * 6);
+                for (int tj = 3 * bj; tj < 3 * bj + 3; ++tj) merge(b, &tileb[((size_t)ti * kTileN + tj) * 6]);
+            const int si = bi / kBlocksPerSuper, sj = bj / kBlocksPerSuper;
+            float* g = gb + (size_t)(si * kSuperN + sj) * kG16 + 6 * (kBlocksPerSuper * (bi % kBlocksPerSuper) + bj % kBlocksPerSuper);
+            for (int a = 0; a < 6; ++a) g[a] = b[a];
         }
     for (int si = 0; si < kSuperN; ++si)
         for (int sj = 0; sj < kSuperN; ++sj) {
-            float* b = bs + ((size_t)si * kSuperN + sj) * 6;
+            float* b = gs + 6 * (si * kSuperN + sj);
             empty(b);
             for (int bi = kBlocksPerSuper * si; bi < std::min(kBlockN, kBlocksPerSuper * (si + 1)); ++bi)
                 for (int bj = kBlocksPerSuper * sj; bj < std::min(kBlockN, kBlocksPerSuper * (sj + 1)); ++bj)
-                    merge(b, bb + ((size_t)bi * kBlockN + bj) * 6);
+                    merge(b, &blockb[((size_t)bi * kBlockN + bj) * 6]);
         }
 }
 
@@ -434,10 +449,10 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads) {
     // max(1, h^2) exceeds both by more than an order of magnitude.
     S.cull_widen = 1e-5 * scale * std::max(1.0, hmax * hmax);
 
-    S.box_cell.assign((size_t)n * kCells * kCells * 6, 0.0f);
-    S.box_tile.assign((size_t)n * kTileN * kTileN * 6, 0.0f);
-    S.box_block.assign((size_t)n * kBlockN * kBlockN * 6, 0.0f);
-    S.box_super.assign((size_t)n * kSuperN * kSuperN * 6, 0.0f);
+    S.box_cell.assign((size_t)n * kCellFloats, 0.0f);
+    S.box_tile.assign((size_t)n * kTileFloats, 0.0f);
+    S.box_block.assign((size_t)n * kBlockFloats, 0.0f);
+    S.box_super.assign((size_t)n * kSuperFloats, 0.0f);
     parallel_for(n, threads, [&](int64_t k) { build_pyramid(S, k, S.cull_widen); });
 
     // BVH (RayTracingDriver.hpp:375-384 -> BVH(objects), leaf threshold 4)

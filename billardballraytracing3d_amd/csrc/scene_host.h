@@ -14,6 +14,18 @@ constexpr int kCoarseN = 100;                         // SurfacePatch.hpp:117
 constexpr int kCells = kCoarseN - 1;                  // 99 cells per side
 constexpr int kTileN = 33, kBlockN = 11, kSuperN = 3; // culling pyramid: 3x3 cells/tile, 3x3 tiles/block,
 constexpr int kBlocksPerSuper = 4;                    // 4x4 blocks per super (last row/col: 3)
+// The pyramid is stored children-contiguous so that one level of one parent is
+// a single aligned chunk: a group of 9 boxes takes 56 floats (54 + 2 pad), a
+// group of 16 boxes 96 floats.  Per patch: supers [56], blocks [9][96] (slot
+// 4*bi'+bj' within the super, empty boxes where the grid ends), tiles
+// [121][56] (by block bi*11+bj, slot 3*ti'+tj'), cells [1089][56] (by tile
+// ti*33+tj, slot 3*i'+j').  A box is lo xyz, hi xyz; an empty box is
+// (+inf, -inf) and never passes the slab test.
+constexpr int kG9 = 56, kG16 = 96;
+constexpr size_t kSuperFloats = kG9;
+constexpr size_t kBlockFloats = 9 * kG16;
+constexpr size_t kTileFloats = (size_t)kBlockN * kBlockN * kG9;
+constexpr size_t kCellFloats = (size_t)kTileN * kTileN * kG9;
 constexpr int kMaxNodes1D = 16;                       // largest imax / jmax the kernels accept
 
 enum : int8_t { kEscaped = BBRT_ESCAPED, kTrapped = BBRT_TRAPPED, kError = BBRT_ERROR };
@@ -53,10 +65,10 @@ struct HostScene {
     std::vector<double> fields;     // 12 * total nodes
     std::vector<double> cxyz;       // [n][100*100][3]
     std::vector<double> cell;       // [n][99*99][4] = sagitta, normal xyz
-    std::vector<float> box_cell;    // [n][99*99][6]
-    std::vector<float> box_tile;    // [n][33*33][6]
-    std::vector<float> box_block;   // [n][11*11][6]
-    std::vector<float> box_super;   // [n][3*3][6]
+    std::vector<float> box_cell;    // [n][kCellFloats]
+    std::vector<float> box_tile;    // [n][kTileFloats]
+    std::vector<float> box_block;   // [n][kBlockFloats]
+    std::vector<float> box_super;   // [n][kSuperFloats]
     std::vector<double> pbox;       // [n][6] inflated patch boxes
     std::vector<double> pcent;      // [n][3]
     std::vector<BvhNode> bvh;

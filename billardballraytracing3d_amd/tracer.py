@@ -27,6 +27,7 @@ class TraceResult:
     final_od: np.ndarray     # float64 [n, 6]
     counters: np.ndarray     # int64 [16]
     hit_count: Optional[np.ndarray] = None
+    ray_work: Optional[np.ndarray] = None   # uint32 [n, 4]
 
     def counter(self, name: str) -> int:
         return int(self.counters[L.CTR[name]])
@@ -106,7 +107,8 @@ class GpuScene:
         return xyz, sag, cn
 
     def trace(self, origins, dirs, max_bounces: int, mode: int = L.NEWTON_CONVERGED,
-              record_k: Optional[int] = None, flags: int = 0, hit_count: bool = False, block: int = 0) -> TraceResult:
+              record_k: Optional[int] = None, flags: int = 0, hit_count: bool = False, block: int = 0,
+              ray_work: bool = False, waves_per_eu: int = 0) -> TraceResult:
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
         d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
         if o.shape != d.shape:
@@ -116,11 +118,12 @@ class GpuScene:
         res = TraceResult(np.zeros(n, np.int8), np.zeros(n, np.int8), np.zeros(n, np.int32), np.zeros(n, np.int32),
                           np.zeros((n, K), np.int32), np.zeros((n, K, 3)), np.zeros((n, K), np.uint8),
                           np.zeros((n, 6)), np.zeros(L.NCOUNTERS, np.int64),
-                          np.zeros(self.n_patches, np.int64) if hit_count else None)
-        p = L.Params(mode, L.SEED_TRIANGLE, max_bounces, K, flags, block)
+                          np.zeros(self.n_patches, np.int64) if hit_count else None,
+                          np.zeros((n, 4), np.uint32) if ray_work else None)
+        p = L.Params(mode, L.SEED_TRIANGLE, max_bounces, K, flags, block, waves_per_eu)
         out = L.Outputs(L.vptr(res.status), L.vptr(res.err_kind), L.vptr(res.bounces), L.vptr(res.err_patch),
                         L.vptr(res.hit_patch), L.vptr(res.hit_xyz), L.vptr(res.iters), L.vptr(res.final_od),
-                        L.vptr(res.hit_count))
+                        L.vptr(res.hit_count), L.vptr(res.ray_work))
         rc = L.lib().bbrt_trace(self.h, L.ptr(o, L._d), L.ptr(d, L._d), C.c_int64(n), C.byref(p), C.byref(out),
                                 L.ptr(res.counters, L._i64))
         L.check(rc, "bbrt_trace")
@@ -128,12 +131,12 @@ class GpuScene:
 
     def trace_device(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
                      mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0, stream: int = 0,
-                     block: int = 0):
+                     block: int = 0, waves_per_eu: int = 0):
         """Trace with device-resident inputs/outputs (raw device addresses). Returns (counters, kernel_ms[3])."""
-        p = L.Params(mode, L.SEED_TRIANGLE, max_bounces, record_k, flags, block)
+        p = L.Params(mode, L.SEED_TRIANGLE, max_bounces, record_k, flags, block, waves_per_eu)
         out = L.Outputs(*[C.c_void_p(out_ptrs.get(k) or 0) if out_ptrs.get(k) else None
                           for k in ("status", "err_kind", "bounces", "err_patch", "hit_patch", "hit_xyz",
-                                    "newton_iters", "final_od", "hit_count")])
+                                    "newton_iters", "final_od", "hit_count", "ray_work")])
         counters = np.zeros(L.NCOUNTERS, np.int64)
         kms = np.zeros(3)
         rc = L.lib().bbrt_trace_device(self.h, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), C.c_int64(n),

@@ -62,7 +62,9 @@ enum { BBRT_ERR_NONE = 0, BBRT_ERR_SINGULAR = 1, BBRT_ERR_NOCONV = 2, BBRT_ERR_D
        BBRT_ERR_BACKFACE = 4, BBRT_ERR_ZERO_DIR = 5, BBRT_ERR_STACK = 6 };
 enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute force (all 58,806 tests) */
        BBRT_FLAG_COUNT_DIRECT_HIT = 2u, /* hit_count also counts each ray's first hit (PlaneSourceDriver.hpp:101) */
-       BBRT_FLAG_TIME_KERNELS = 4u };   /* record HIP events around every bounce-kernel launch */
+       BBRT_FLAG_TIME_KERNELS = 4u,     /* record HIP events around every bounce-kernel launch */
+       BBRT_FLAG_WAVEFRONT = 8u };      /* one launch per bounce with live-ray compaction instead of the
+                                           persistent kernel with a device ray queue */
 
 typedef struct {
     int32_t newton_mode;   /* BBRT_NEWTON_* */
@@ -71,6 +73,8 @@ typedef struct {
     int32_t record_k;      /* bounces recorded per ray in hit_patch / hit_xyz / newton_iters */
     uint32_t flags;        /* BBRT_FLAG_* */
     int32_t block;         /* threads per block of the bounce kernel; 0 = library default */
+    int32_t waves_per_eu;  /* occupancy target of the persistent kernel: 0 = library default, or 2, 3, 4
+                              (the compiler keeps registers within that occupancy, spilling if needed) */
 } bbrt_params;
 
 /* Outputs; any pointer may be NULL.  Host memory for bbrt_trace, device
@@ -85,7 +89,10 @@ typedef struct {
  *                          (an escaped ray's tail point is origin + dir * scene diagonal,
  *                          RayTracingDriver.hpp:198-203)
  *   hit_count              [n_patches] int64  PlaneSourceDriver hit counts (accumulated,
- *                          not cleared), first hit skipped unless BBRT_FLAG_COUNT_DIRECT_HIT */
+ *                          not cleared), first hit skipped unless BBRT_FLAG_COUNT_DIRECT_HIT
+ * Record slots past a ray's bounce count are written only by bbrt_trace (which
+ * pads them with -1 / 0); bbrt_trace_device leaves them untouched, like the
+ * reference's RayPath vectors, which hold bounce_count entries. */
 typedef struct {
     int8_t* status;
     int8_t* err_kind;
@@ -96,6 +103,8 @@ typedef struct {
     uint8_t* newton_iters;
     double* final_od;
     int64_t* hit_count;
+    uint32_t* ray_work;   /* [n][4]: BVH node tests, culling box tests, Moller-Trumbore tests and
+                             Newton iterations spent on each ray, summed over its bounces */
 } bbrt_outputs;
 
 /* Work counters returned by the trace calls (int64[BBRT_NCOUNTERS]). */

@@ -77,6 +77,18 @@ def test_trace_matches_oracle(name, mode):
 
 
 @pytest.mark.parametrize("name", ["c1", "c2s"])
+def test_wavefront_schedule_matches_persistent(name):
+    sc, o, d, mb = _scene(name)
+    gs = _gpu_scene(name, sc)
+    print(f"[schedule] scene={name}", flush=True)
+    a = gs.trace(o, d, mb, mode=L.NEWTON_CONVERGED)
+    b = gs.trace(o, d, mb, mode=L.NEWTON_CONVERGED, flags=L.FLAG_WAVEFRONT)
+    assert_same(a, b, name + " persistent vs wavefront")
+    for k in ("bounces", "bvh_node_tests", "candidates", "cull_box_tests", "mt_tests", "newton_iters"):
+        assert a.counter(k) == b.counter(k), k
+
+
+@pytest.mark.parametrize("name", ["c1", "c2s"])
 def test_culling_equals_brute_force(name):
     sc, o, d, mb = _scene(name)
     gs = _gpu_scene(name, sc)
