@@ -536,36 +536,43 @@ __device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32
         const bool ok = root_live ? true : box_static(o, inv, root, tm);
         cur = make_ent(0, root, ok, tm);
     }
+    // While-while form: the inner loop walks internal nodes (cheap) until the
+    // lane stands on a leaf to intersect or runs out of nodes; leaves are then
+    // intersected with the whole wave in step, instead of every lane paying
+    // for a patch intersection whenever any lane reaches a leaf.
     for (;;) {
-        c.bvh_tests++;
-        const Ent e = cur;
-        bool expand = e.idx >= 0 && !(e.tmin > best_t);
-        if (expand && e.meta < 0) {
-            const int v = -1 - e.meta;
-            const int start = v >> 3, count = v & 7;
-            for (int q = 0; q < count; ++q) {
-                const int32_t obj = S.leaf[start + q];
-                const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
-                if (h.status < 0) { err_obj = obj; return -h.status; }
-                if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
+        bool at_leaf = false;
+        for (;;) {
+            c.bvh_tests++;
+            const bool live = cur.idx >= 0 && !(cur.tmin > best_t);
+            if (live && cur.meta < 0) { at_leaf = true; break; }
+            if (!live) {
+                if (sp == 0) break;
+                cur = stack[--sp];
+                continue;
             }
-            expand = false;
+            const int32_t left = cur.idx + 1, right = cur.meta >> 2;
+            const BvhNode nl = S.bvh[left];
+            const BvhNode nr = S.bvh[right];
+            double tl = 0.0, tr = 0.0;
+            const bool okl = box_static(o, inv, nl, tl);
+            const bool okr = box_static(o, inv, nr, tr);
+            if (sp + 1 > kStackCap) return bbrt::kErrStack;
+            const Ent el = make_ent(left, nl, okl, tl), er = make_ent(right, nr, okr, tr);
+            if (neg[cur.meta & 3]) { stack[sp++] = el; cur = er; }
+            else { stack[sp++] = er; cur = el; }
         }
-        if (!expand) {
-            if (sp == 0) break;
-            cur = stack[--sp];
-            continue;
+        if (!at_leaf) break;
+        const int v = -1 - cur.meta;
+        const int start = v >> 3, count = v & 7;
+        for (int q = 0; q < count; ++q) {
+            const int32_t obj = S.leaf[start + q];
+            const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
+            if (h.status < 0) { err_obj = obj; return -h.status; }
+            if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
         }
-        const int32_t left = e.idx + 1, right = e.meta >> 2;
-        const BvhNode nl = S.bvh[left];
-        const BvhNode nr = S.bvh[right];
-        double tl = 0.0, tr = 0.0;
-        const bool okl = box_static(o, inv, nl, tl);
-        const bool okr = box_static(o, inv, nr, tr);
-        if (sp + 1 > kStackCap) return bbrt::kErrStack;
-        const Ent el = make_ent(left, nl, okl, tl), er = make_ent(right, nr, okr, tr);
-        if (neg[e.meta & 3]) { stack[sp++] = el; cur = er; }
-        else { stack[sp++] = er; cur = el; }
+        if (sp == 0) break;
+        cur = stack[--sp];
     }
     return 0;
 }
