@@ -66,6 +66,7 @@ struct DevScene {
     const float* box_tile;
     const float* box_block;
     const float* box_super;
+    const float* box_patch;
     const BvhNode* bvh;
     const int32_t* leaf;
     double diag;
@@ -105,9 +106,11 @@ __device__ __forceinline__ V3 unit3(V3 v) {
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 
+// Per-lane work counts (32-bit: one lane's share of one trace fits easily);
+// reduced over the wave and added to the 64-bit totals once per kernel.
 struct Counters {
-    unsigned long long bvh_tests, candidates, cull_tests, mt_tests, newton_calls, newton_iters, newton_flops,
-        polish_fail, bounces;
+    uint32_t bvh_tests, candidates, cull_tests, mt_tests, newton_calls, newton_iters, newton_flops, polish_fail,
+        bounces;
 };
 
 // MollerTrumbore.hpp:32-64 (t only; the point and normal are unused by SurfacePatch).
@@ -300,10 +303,31 @@ __device__ __forceinline__ void basis(const double* __restrict__ nodes, const do
         if (i < cnt) b[i] /= denom;
 }
 
-// SurfacePatch.hpp:542-555
+// SurfacePatch.hpp:542-555.  For a fixed node count all values are loaded
+// first (one round trip), then combined in the reference's order, skipping
+// rows whose u-basis value is zero exactly as the reference does.
 template <int N>
 __device__ __forceinline__ double field(const double* lu, const double* lv, const double* __restrict__ f, int im,
                                         int jm) {
+    if constexpr (N > 0) {
+        // keep the scheduler from hoisting the next field's loads above this one
+        __builtin_amdgcn_sched_barrier(0);
+        double v[N * N];
+#pragma unroll
+        for (int k = 0; k < N * N; ++k) v[k] = f[k];
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double li = lu[i];
+            if (li != 0.0) {
+                double row = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j) row += lv[j] * v[i * N + j];
+                sum += li * row;
+            }
+        }
+        return sum;
+    }
     constexpr int M = N > 0 ? N : bbrt::kMaxNodes1D;
     const int ci = N > 0 ? N : im, cj = N > 0 ? N : jm;
     double sum = 0.0;
@@ -368,9 +392,9 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
         return solve3(J, Fv, delta);
     };
     // flops per iteration and per final evaluation (N = nodes per side)
-    const unsigned long long Ni = (unsigned long long)im;
-    const unsigned long long I_it = 8 * Ni + 18 * Ni * (Ni + 1) + 77;
-    const unsigned long long F_fin = 8 * Ni + 12 * Ni * (Ni + 1) + 9;
+    const uint32_t Ni = (uint32_t)im;
+    const uint32_t I_it = 8 * Ni + 18 * Ni * (Ni + 1) + 77;
+    const uint32_t F_fin = 8 * Ni + 12 * Ni * (Ni + 1) + 9;
     int iters = 0;
     double step_norm = INFINITY;
     for (int it = 0; it < kNewtonMaxIters; ++it) {
@@ -449,7 +473,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 __device__ __forceinline__ void flush_counters(const Counters& c, unsigned long long* ctr) {
     const unsigned long long v[9] = { c.bvh_tests, c.candidates, c.cull_tests, c.mt_tests, c.newton_calls,
-                                      c.newton_iters, c.newton_flops, c.polish_fail, c.bounces };
+                                      c.newton_iters, c.newton_flops, c.polish_fail, c.bounces };  // widened
     const int slot[9] = { bbrt::kCtrBvh, bbrt::kCtrCand, bbrt::kCtrCull, bbrt::kCtrMt, bbrt::kCtrNewtonCalls,
                           bbrt::kCtrNewtonIters, bbrt::kCtrNewtonFlops, bbrt::kCtrPolishFail, bbrt::kCtrBounces };
 #pragma unroll
@@ -565,11 +589,34 @@ __device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32
         if (!at_leaf) break;
         const int v = -1 - cur.meta;
         const int start = v >> 3, count = v & 7;
-        for (int q = 0; q < count; ++q) {
-            const int32_t obj = S.leaf[start + q];
-            const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
-            if (h.status < 0) { err_obj = obj; return -h.status; }
-            if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
+        for (int q0 = 0; q0 < count; q0 += 4) {
+            // Batch prefilter: the conservative box of each object's whole
+            // culling pyramid, loaded together.  An object whose box the ray
+            // does not enter has no stage-1 hit: a miss, with no further loads.
+            int32_t ids[4];
+            uint32_t enter = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ids[q] = (q0 + q < count) ? S.leaf[start + q0 + q] : -1;
+            if (cull) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (ids[q] >= 0) {
+                        const float4* b4 = reinterpret_cast<const float4*>(S.box_patch + (size_t)ids[q] * 8);
+                        const float4 a = b4[0], b = b4[1];
+                        const float bx[6] = { a.x, a.y, a.z, a.w, b.x, b.y };
+                        if (box32(bx, r32)) enter |= 1u << q;
+                    }
+                c.cull_tests += min(4, count - q0);
+            } else {
+                enter = 0xFu;
+            }
+            for (int q = 0; q < 4 && q0 + q < count; ++q) {
+                const int32_t obj = ids[q];
+                if (!(enter & (1u << q))) { c.candidates++; continue; }
+                const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
+                if (h.status < 0) { err_obj = obj; return -h.status; }
+                if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
+            }
         }
         if (sp == 0) break;
         cur = stack[--sp];
@@ -579,9 +626,11 @@ __device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32
 
 // One bounce of ray r: RayTracingDriver.hpp:186-237.  Returns true while the
 // ray continues; o and d are advanced in place.
+template <bool WORK>
 __device__ bool bounce_once(const DevScene& S, const Outs& O, int32_t r, V3& o, V3& d, int bounce, int max_bounces,
                             int record_k, int mode, uint32_t flags, bool root_live, Counters& c) {
-    const Counters c0 = c;
+    Counters c0;
+    if constexpr (WORK) c0 = c;
     const bool finite = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(d.x) && isfinite(d.y) &&
                         isfinite(d.z);
     const bool cull = !(flags & BBRT_FLAG_NO_CULL) && finite && fabs(o.x) <= S.cull_limit &&
@@ -631,7 +680,7 @@ __device__ bool bounce_once(const DevScene& S, const Outs& O, int32_t r, V3& o, 
             alive = true;
         }
     }
-    if (O.ray_work) {
+    if constexpr (WORK) {
         uint32_t* w = O.ray_work + 4 * (size_t)r;
         w[0] += (uint32_t)(c.bvh_tests - c0.bvh_tests);
         w[1] += (uint32_t)(c.cull_tests - c0.cull_tests);
@@ -700,7 +749,7 @@ __global__ void k_init(const double* __restrict__ origins, const double* __restr
 // registers, and take the next ray when one ends.  Every wave leaves once the
 // list is exhausted and its lanes' rays have ended (each ray ends within
 // max_bounces bounces).
-template <int W>
+template <int W, bool WORK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k_persist(DevScene S, RayState R, const int32_t* __restrict__ live,
                                                  const int32_t* __restrict__ n_live_p, int32_t* head, Outs O,
                                                  int max_bounces, int record_k, int mode, uint32_t flags,
@@ -732,7 +781,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k
         }
         if (__ballot(r >= 0) == 0) break;
         if (r >= 0) {
-            if (bounce_once(S, O, r, o, d, b, max_bounces, record_k, mode, flags, b == 0, c)) ++b;
+            if (bounce_once<WORK>(S, O, r, o, d, b, max_bounces, record_k, mode, flags, b == 0, c)) ++b;
             else r = -1;
         }
     }
@@ -753,7 +802,8 @@ __global__ void __launch_bounds__(256) k_bounce(DevScene S, RayState R, const in
         r = live_in[g];
         V3 o = { R.o[3 * r], R.o[3 * r + 1], R.o[3 * r + 2] };
         V3 d = { R.d[3 * r], R.d[3 * r + 1], R.d[3 * r + 2] };
-        alive = bounce_once(S, O, r, o, d, bounce, max_bounces, record_k, mode, flags, bounce == 0, c);
+        alive = O.ray_work ? bounce_once<true>(S, O, r, o, d, bounce, max_bounces, record_k, mode, flags, bounce == 0, c)
+                           : bounce_once<false>(S, O, r, o, d, bounce, max_bounces, record_k, mode, flags, bounce == 0, c);
         if (alive) {
             R.o[3 * r] = o.x; R.o[3 * r + 1] = o.y; R.o[3 * r + 2] = o.z;
             R.d[3 * r] = d.x; R.d[3 * r + 1] = d.y; R.d[3 * r + 2] = d.z;
@@ -913,11 +963,12 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     if (!wavefront) {
         if (p->max_bounces > 0) {
             const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : kDefaultWavesPerEu;
-            decltype(&k_persist<1>) kern = nullptr;
+            decltype(&k_persist<1, false>) kern = nullptr;
+            const bool work = O.ray_work != nullptr;
             switch (wpe) {
-                case 1: case 2: kern = &k_persist<1>; break;
-                case 3: kern = &k_persist<3>; break;
-                case 4: kern = &k_persist<4>; break;
+                case 1: case 2: kern = work ? &k_persist<1, true> : &k_persist<1, false>; break;
+                case 3: kern = work ? &k_persist<3, true> : &k_persist<3, false>; break;
+                case 4: kern = work ? &k_persist<4, true> : &k_persist<4, false>; break;
                 default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
             }
             int per_cu = 0;
@@ -1055,6 +1106,7 @@ int bbrt_scene_create(const bbrt_patch_set* ps, bbrt_scene* out) {
         D.box_tile = dev_upload(H.box_tile, s->allocs, s->bytes);
         D.box_block = dev_upload(H.box_block, s->allocs, s->bytes);
         D.box_super = dev_upload(H.box_super, s->allocs, s->bytes);
+        D.box_patch = dev_upload(H.box_patch, s->allocs, s->bytes);
         D.bvh = dev_upload(H.bvh, s->allocs, s->bytes);
         D.leaf = dev_upload(H.leaf, s->allocs, s->bytes);
         D.diag = H.diag;

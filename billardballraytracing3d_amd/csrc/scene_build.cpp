@@ -242,6 +242,9 @@ void build_pyramid(HostScene& S, int64_t k, double widen) {
     for (int b = 0; b < kBlockN * kBlockN; ++b) empty_group(gt + (size_t)b * kG9, 9);
     for (int s = 0; s < kSuperN * kSuperN; ++s) empty_group(gb + (size_t)s * kG16, 16);
     empty_group(gs, 9);
+    float* gp = &S.box_patch[(size_t)k * 8];
+    for (int a = 0; a < 3; ++a) { gp[a] = -inf; gp[3 + a] = inf; }   // full box: always entered
+    gp[6] = gp[7] = 0.0f;
     if (!S.meta[k].cull) return;   // never consulted: such a patch is tested exhaustively
 
     std::vector<float> cellb((size_t)kCells * kCells * 6), tileb((size_t)kTileN * kTileN * 6),
@@ -289,6 +292,7 @@ void build_pyramid(HostScene& S, int64_t k, double widen) {
             float* g = gb + (size_t)(si * kSuperN + sj) * kG16 + 6 * (kBlocksPerSuper * (bi % kBlocksPerSuper) + bj % kBlocksPerSuper);
             for (int a = 0; a < 6; ++a) g[a] = b[a];
         }
+    empty(gp);
     for (int si = 0; si < kSuperN; ++si)
         for (int sj = 0; sj < kSuperN; ++sj) {
             float* b = gs + 6 * (si * kSuperN + sj);
@@ -296,6 +300,7 @@ void build_pyramid(HostScene& S, int64_t k, double widen) {
             for (int bi = kBlocksPerSuper * si; bi < std::min(kBlockN, kBlocksPerSuper * (si + 1)); ++bi)
                 for (int bj = kBlocksPerSuper * sj; bj < std::min(kBlockN, kBlocksPerSuper * (sj + 1)); ++bj)
                     merge(b, &blockb[((size_t)bi * kBlockN + bj) * 6]);
+            merge(gp, b);
         }
 }
 
@@ -453,6 +458,7 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads) {
     S.box_tile.assign((size_t)n * kTileFloats, 0.0f);
     S.box_block.assign((size_t)n * kBlockFloats, 0.0f);
     S.box_super.assign((size_t)n * kSuperFloats, 0.0f);
+    S.box_patch.assign((size_t)n * 8, 0.0f);
     parallel_for(n, threads, [&](int64_t k) { build_pyramid(S, k, S.cull_widen); });
 
     // BVH (RayTracingDriver.hpp:375-384 -> BVH(objects), leaf threshold 4)

@@ -69,6 +69,7 @@ struct HostScene {
     std::vector<float> box_tile;    // [n][kTileFloats]
     std::vector<float> box_block;   // [n][kBlockFloats]
     std::vector<float> box_super;   // [n][kSuperFloats]
+    std::vector<float> box_patch;   // [n][8]: union of the super boxes (+-inf when the patch is not culled)
     std::vector<double> pbox;       // [n][6] inflated patch boxes
     std::vector<double> pcent;      // [n][3]
     std::vector<BvhNode> bvh;
