@@ -1,0 +1,870 @@
+// drivers.hpp -- C++ drop-in for the reference's host interfaces, over the C ABI
+// of libbbrt_hip.so (include/bbrt.h).  Header-only; link with -lbbrt_hip.
+//
+// Same class names, signatures, argument meaning and error behaviour as the
+// reference's headers, with the per-ray trace loop replaced by one batched
+// GPU call:
+//   ScatObject, Vec3, Ray, AABB, Hit         include/ScatObject.hpp:10-89
+//   SurfacePatch (file format, data,
+//     sample_grid, max_sagitta)               include/ScatObjects/SurfacePatch.hpp:52-371
+//   RayTracingDriver                          include/RayTracingDriver.hpp:65-425
+//   PlaneSourceDriver                         include/PlaneSourceDriver.hpp:55-244
+//   BounceMapDriver                           include/BounceMapDriver.hpp:61-520
+// There is no CPU intersection path: SurfacePatch::intersect throws, every
+// trace runs the HIP kernels.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../bbrt.h"
+
+namespace bbrt_cpp {
+
+// --- value types (ScatObject.hpp:10-47) ----------------------------------
+struct Vec3 { double x, y, z; };
+struct Ray { Vec3 origin; Vec3 dir; };
+struct AABB {
+    Vec3 min{ std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
+              std::numeric_limits<double>::infinity() };
+    Vec3 max{ -std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity(),
+              -std::numeric_limits<double>::infinity() };
+};
+class ScatObject;
+struct Hit {
+    double t;
+    Vec3 point;
+    Vec3 normal;
+    const ScatObject* object = nullptr;
+};
+
+// ScatObject.hpp:55-89
+class ScatObject {
+public:
+    virtual ~ScatObject() = default;
+    const AABB& bounding_box() const { return box_; }
+    const Vec3& centroid() const { return centroid_; }
+    virtual std::optional<Hit> intersect(const Ray& ray) const = 0;
+    virtual double self_intersect_padding() const { return 0.0; }
+protected:
+    AABB box_;
+    Vec3 centroid_{ 0, 0, 0 };
+};
+
+inline Vec3 vec_sub(const Vec3& a, const Vec3& b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+inline Vec3 vec_cross(const Vec3& a, const Vec3& b) {
+    return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
+}
+inline double vec_dot(const Vec3& a, const Vec3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 vec_normalize(const Vec3& v) {
+    double len = std::sqrt(vec_dot(v, v));
+    return { v.x / len, v.y / len, v.z / len };
+}
+
+namespace detail {
+inline void check(int rc, const char* what) {
+    if (rc != 0) throw std::runtime_error(std::string(what) + ": " + bbrt_last_error());
+}
+// std::min/std::max of the reference: a NaN second argument is ignored.
+inline double lesser(double a, double b) { return (b < a) ? b : a; }
+inline double greater(double a, double b) { return (a < b) ? b : a; }
+
+// PatchInterpolation.hpp:25-52
+inline std::vector<double> barycentric_weights(const std::vector<double>& x) {
+    const int n = (int)x.size();
+    std::vector<double> w(n, 1.0);
+    for (int j = 0; j < n; ++j) {
+        double p = 1.0;
+        for (int i = 0; i < n; ++i)
+            if (i != j) p *= (x[j] - x[i]);
+        w[j] = 1.0 / p;
+    }
+    return w;
+}
+
+// First-form tensor interpolation (PatchInterpolation.hpp:54-159) with the two
+// matrix products summed over their inner index upwards from 0.0.
+inline void interpolate_2d(const std::vector<double>& xn, const std::vector<double>& yn, const std::vector<double>& xw,
+                           const std::vector<double>& yw, const std::vector<double>& F, const std::vector<double>& x,
+                           const std::vector<double>& y, double* out) {
+    const int N = (int)xn.size(), M = (int)yn.size(), P = (int)x.size(), Q = (int)y.size();
+    auto rows = [](const std::vector<double>& nodes, const std::vector<double>& w, const std::vector<double>& pts,
+                   std::vector<double>& r, std::vector<double>& scale) {
+        const int n = (int)nodes.size(), m = (int)pts.size();
+        r.assign((size_t)m * n, 0.0);
+        scale.assign(m, 1.0);
+        for (int i = 0; i < m; ++i) {
+            double prod = 1.0;
+            int hot = -1;
+            for (int k = 0; k < n; ++k) {
+                const double diff = pts[i] - nodes[k];
+                if (std::fabs(diff) < 1.0e-12) { hot = k; break; }
+                r[(size_t)i * n + k] = w[k] / diff;
+                prod *= diff;
+            }
+            if (hot >= 0) {
+                for (int k = 0; k < n; ++k) r[(size_t)i * n + k] = 0.0;
+                r[(size_t)i * n + hot] = 1.0;
+                scale[i] = 1.0;
+            } else {
+                scale[i] = prod;
+            }
+        }
+    };
+    std::vector<double> rx, sx, ry, sy;
+    rows(xn, xw, x, rx, sx);
+    rows(yn, yw, y, ry, sy);
+    std::vector<double> mid((size_t)P * M);
+    for (int i = 0; i < P; ++i)
+        for (int m = 0; m < M; ++m) {
+            double acc = 0.0;
+            for (int k = 0; k < N; ++k) acc += rx[(size_t)i * N + k] * F[(size_t)k * M + m];
+            mid[(size_t)i * M + m] = acc;
+        }
+    for (int i = 0; i < P; ++i)
+        for (int j = 0; j < Q; ++j) {
+            double acc = 0.0;
+            for (int m = 0; m < M; ++m) acc += mid[(size_t)i * M + m] * ry[(size_t)j * M + m];
+            out[(size_t)i * Q + j] = acc * (sx[i] * sy[j]);
+        }
+}
+}  // namespace detail
+
+// --- SurfacePatch (SurfacePatch.hpp:52-371) --------------------------------
+class SurfacePatch : public ScatObject {
+public:
+    struct Data {
+        int imax = 0, jmax = 0, kmax = 0;
+        long long zoneID = 0;
+        std::vector<double> uNodes, vNodes, uWeights, vWeights;
+        std::vector<double> x, y, z;
+        std::vector<int> mask;
+        std::vector<double> dxdu, dydu, dzdu, dxdv, dydv, dzdv, dS, nuX, nuY, nuZ;
+        std::array<int, 4> edge_flags{ { 0, 0, 0, 0 } };
+        long long point_count() const { return static_cast<long long>(imax) * jmax * kmax; }
+    };
+    struct SampledGrid {
+        int n = 0;
+        std::vector<double> u, v, x, y, z, nx, ny, nz, dxdu, dydu, dzdu;
+        Vec3 point(int i, int j) const { const int k = i * n + j; return { x[k], y[k], z[k] }; }
+        Vec3 normal(int i, int j) const { const int k = i * n + j; return { nx[k], ny[k], nz[k] }; }
+        Vec3 tangent_u(int i, int j) const { const int k = i * n + j; return { dxdu[k], dydu[k], dzdu[k] }; }
+    };
+
+    explicit SurfacePatch(const std::string& path) : data_(load_patch_file(path)) { compute_centroid_and_box(); }
+    SurfacePatch(const std::string& directory, const std::string& file_prefix, long long index)
+        : SurfacePatch(directory + file_prefix + std::to_string(index + 1) + ".txt") {}
+
+    const Data& data() const { return data_; }
+    double max_sagitta() const { return max_sagitta_; }
+    double self_intersect_padding() const override { return max_sagitta_; }
+
+    // The intersection runs on the GPU inside RayTracingDriver; there is no host path.
+    std::optional<Hit> intersect(const Ray&) const override {
+        throw std::logic_error("SurfacePatch::intersect: patches are intersected on the GPU by RayTracingDriver");
+    }
+
+    // SurfacePatch.hpp:199-242
+    SampledGrid sample_grid(int n) const {
+        SampledGrid g;
+        g.n = n;
+        g.u.resize(n); g.v.resize(n);
+        for (int i = 0; i < n; ++i) {
+            const double t = (n == 1) ? 0.0 : -1.0 + 2.0 * static_cast<double>(i) / (n - 1);
+            g.u[i] = t; g.v[i] = t;
+        }
+        const size_t nn = (size_t)n * n;
+        std::vector<double>* outs[9] = { &g.x, &g.y, &g.z, &g.nx, &g.ny, &g.nz, &g.dxdu, &g.dydu, &g.dzdu };
+        const std::vector<double>* ins[9] = { &data_.x, &data_.y, &data_.z, &data_.nuX, &data_.nuY, &data_.nuZ,
+                                              &data_.dxdu, &data_.dydu, &data_.dzdu };
+        for (int f = 0; f < 9; ++f) {
+            outs[f]->resize(nn);
+            detail::interpolate_2d(data_.uNodes, data_.vNodes, data_.uWeights, data_.vWeights, *ins[f], g.u, g.v,
+                                   outs[f]->data());
+        }
+        for (size_t k = 0; k < nn; ++k) {
+            const Vec3 nrm = vec_normalize({ g.nx[k], g.ny[k], g.nz[k] });
+            g.nx[k] = nrm.x; g.ny[k] = nrm.y; g.nz[k] = nrm.z;
+        }
+        return g;
+    }
+
+    // Filled in by the owning driver from bbrt_scene_patch_info (the inflated box and sagitta).
+    void set_prepared(const double* box, double max_sag) {
+        box_.min = { box[0], box[1], box[2] };
+        box_.max = { box[3], box[4], box[5] };
+        max_sagitta_ = max_sag;
+    }
+
+private:
+    Data data_;
+    double max_sagitta_ = 0.0;
+
+    // SurfacePatch.hpp:313-371
+    static Data load_patch_file(const std::string& path) {
+        std::ifstream fin(path);
+        if (!fin.is_open()) throw std::runtime_error("SurfacePatch: cannot open patch file: " + path);
+        Data d;
+        fin >> d.imax >> d.jmax >> d.kmax >> d.zoneID;
+        if (!fin) throw std::runtime_error("SurfacePatch: bad header in patch file: " + path);
+        const long long n = d.point_count();
+        auto read_vec = [&](std::vector<double>& v, long long count) {
+            v.resize(count);
+            for (long long i = 0; i < count; ++i) fin >> v[i];
+        };
+        read_vec(d.uNodes, d.imax);
+        read_vec(d.vNodes, d.jmax);
+        d.uWeights = detail::barycentric_weights(d.uNodes);
+        d.vWeights = detail::barycentric_weights(d.vNodes);
+        read_vec(d.x, n); read_vec(d.y, n); read_vec(d.z, n);
+        d.mask.resize(n);
+        for (long long i = 0; i < n; ++i) fin >> d.mask[i];
+        read_vec(d.dxdu, n); read_vec(d.dydu, n); read_vec(d.dzdu, n);
+        read_vec(d.dxdv, n); read_vec(d.dydv, n); read_vec(d.dzdv, n);
+        read_vec(d.dS, n);
+        read_vec(d.nuX, n); read_vec(d.nuY, n); read_vec(d.nuZ, n);
+        if (!fin) throw std::runtime_error("SurfacePatch: truncated patch file: " + path);
+        fin >> std::ws;
+        if (!fin.eof()) {
+            if (!(fin >> d.edge_flags[0] >> d.edge_flags[1] >> d.edge_flags[2] >> d.edge_flags[3]))
+                throw std::runtime_error("SurfacePatch: malformed sharp-edge data in patch file: " + path);
+        }
+        return d;
+    }
+
+    // SurfacePatch.hpp:380-407 (the box is replaced by the inflated one once the scene is built)
+    void compute_centroid_and_box() {
+        const long long n = data_.point_count();
+        if (n == 0) throw std::runtime_error("SurfacePatch: patch has no points");
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        AABB box;
+        for (long long i = 0; i < n; ++i) {
+            const double px = data_.x[i], py = data_.y[i], pz = data_.z[i];
+            sx += px; sy += py; sz += pz;
+            box.min.x = detail::lesser(box.min.x, px); box.min.y = detail::lesser(box.min.y, py);
+            box.min.z = detail::lesser(box.min.z, pz);
+            box.max.x = detail::greater(box.max.x, px); box.max.y = detail::greater(box.max.y, py);
+            box.max.z = detail::greater(box.max.z, pz);
+        }
+        const double inv_n = 1.0 / static_cast<double>(n);
+        centroid_ = { sx * inv_n, sy * inv_n, sz * inv_n };
+        box_ = box;
+    }
+};
+
+// --- RayTracingDriver (RayTracingDriver.hpp:65-425) --------------------------
+class RayTracingDriver {
+public:
+    struct RayPath {
+        enum class Status { Escaped, Trapped, Error };
+        std::vector<Vec3> points;
+        Status status = Status::Trapped;
+        int bounce_count = 0;
+        std::string error_message;
+        std::vector<const ScatObject*> hit_objects;
+    };
+
+    // Newton mode of the traces (BBRT_NEWTON_REF reproduces the reference's
+    // stopping rule; BBRT_NEWTON_CONVERGED polishes the hit points).
+    int newton_mode = BBRT_NEWTON_REF;
+
+    explicit RayTracingDriver(const std::string& patch_directory) {
+        load_patches(patch_directory);
+        build_scene();
+    }
+    RayTracingDriver(const std::string& directory, const std::string& file_prefix, int first_index_1based,
+                     int last_index_1based) {
+        load_patches_range(directory, file_prefix, first_index_1based, last_index_1based);
+        build_scene();
+    }
+    ~RayTracingDriver() { if (scene_) bbrt_scene_destroy(scene_); }
+    RayTracingDriver(const RayTracingDriver&) = delete;
+    RayTracingDriver& operator=(const RayTracingDriver&) = delete;
+
+    // RayTracingDriver.hpp:119-136
+    void set_rays(std::vector<Vec3> sources, std::vector<Vec3> directions) {
+        if (sources.size() != directions.size())
+            throw std::invalid_argument(
+                "RayTracingDriver::set_rays: sources and directions must be the same length (paired 1:1) — got " +
+                std::to_string(sources.size()) + " sources and " + std::to_string(directions.size()) + " directions");
+        for (size_t i = 0; i < directions.size(); ++i) {
+            const Vec3& d = directions[i];
+            if (d.x * d.x + d.y * d.y + d.z * d.z < 1e-300)
+                throw std::invalid_argument("RayTracingDriver::set_rays: direction " + std::to_string(i) +
+                                            " has zero length");
+        }
+        sources_ = std::move(sources);
+        directions_ = std::move(directions);
+    }
+
+    // RayTracingDriver.hpp:143-150: every configured ray, in one GPU batch.
+    void run(int max_bounces = 50) { paths_ = trace_batch(sources_, directions_, max_bounces); }
+
+    const std::vector<RayPath>& paths() const { return paths_; }
+    const std::vector<std::unique_ptr<SurfacePatch>>& patches() const { return patches_; }
+
+    long long patch_index_of(const ScatObject* object) const {
+        auto it = index_by_ptr_.find(object);
+        return it != index_by_ptr_.end() ? static_cast<long long>(it->second) : -1;
+    }
+
+    // Index of the patch loaded from `path` (same file), or -1.
+    long long patch_index_of_path(const std::string& path) const {
+        namespace fs = std::filesystem;
+        std::error_code ec;
+        for (size_t k = 0; k < paths_loaded_.size(); ++k)
+            if (fs::equivalent(paths_loaded_[k], path, ec)) return (long long)k;
+        return -1;
+    }
+
+    // Raw batched trace: status, error kind, bounce count and failing patch per
+    // ray; records only when record_k > 0.  counters may be NULL.
+    void trace_raw(const std::vector<double>& o, const std::vector<double>& d, int max_bounces, int record_k,
+                   std::vector<int8_t>& status, std::vector<int8_t>& kind, std::vector<int32_t>& bounces,
+                   std::vector<int32_t>& err_patch, std::vector<int32_t>* hit_patch, std::vector<double>* hit_xyz,
+                   std::vector<double>* final_od, int64_t* hit_count = nullptr, bool count_direct_hit = false,
+                   int64_t* counters = nullptr) const {
+        const size_t n = o.size() / 3;
+        status.resize(n); kind.resize(n); bounces.resize(n); err_patch.resize(n);
+        bbrt_params p{};
+        p.newton_mode = newton_mode;
+        p.seed_mode = BBRT_SEED_TRIANGLE;
+        p.max_bounces = std::max(0, max_bounces);
+        p.record_k = record_k;
+        p.flags = count_direct_hit ? BBRT_FLAG_COUNT_DIRECT_HIT : 0u;
+        bbrt_outputs q{};
+        q.status = status.data(); q.err_kind = kind.data(); q.bounces = bounces.data(); q.err_patch = err_patch.data();
+        if (record_k > 0 && hit_patch) { hit_patch->resize(n * record_k); q.hit_patch = hit_patch->data(); }
+        if (record_k > 0 && hit_xyz) { hit_xyz->resize(n * record_k * 3); q.hit_xyz = hit_xyz->data(); }
+        if (final_od) { final_od->resize(6 * n); q.final_od = final_od->data(); }
+        q.hit_count = hit_count;
+        detail::check(bbrt_trace(scene_, o.data(), d.data(), (int64_t)n, &p, &q, counters), "bbrt_trace");
+    }
+
+    // RayTracingDriver.hpp:172-242 for one ray (a batch of one on the GPU).
+    RayPath trace_ray(const Vec3& origin, const Vec3& direction, int max_bounces) const {
+        return trace_batch({ origin }, { direction }, max_bounces).front();
+    }
+
+    // The batched form the drivers use: ray i is sources[i] -> directions[i].
+    // Zero-length directions come back as Error paths (like trace_ray).
+    std::vector<RayPath> trace_batch(const std::vector<Vec3>& sources, const std::vector<Vec3>& directions,
+                                     int max_bounces, int64_t* hit_count = nullptr, bool count_direct_hit = false,
+                                     int64_t* counters = nullptr) const {
+        const size_t n = sources.size();
+        if (directions.size() != n) throw std::invalid_argument("RayTracingDriver::trace_batch: size mismatch");
+        std::vector<RayPath> out(n);
+        if (n == 0) return out;
+        if (max_bounces < 0) max_bounces = 0;
+        const int K = std::max(1, max_bounces);
+        std::vector<double> o(3 * n), d(3 * n), hit_xyz, final_od;
+        for (size_t i = 0; i < n; ++i) {
+            o[3 * i] = sources[i].x; o[3 * i + 1] = sources[i].y; o[3 * i + 2] = sources[i].z;
+            d[3 * i] = directions[i].x; d[3 * i + 1] = directions[i].y; d[3 * i + 2] = directions[i].z;
+        }
+        std::vector<int8_t> status, kind;
+        std::vector<int32_t> bounces, err_patch, hit_patch;
+        trace_raw(o, d, max_bounces, K, status, kind, bounces, err_patch, &hit_patch, &hit_xyz, &final_od, hit_count,
+                  count_direct_hit, counters);
+        for (size_t i = 0; i < n; ++i) {
+            RayPath& path = out[i];
+            path.points.push_back(sources[i]);
+            path.bounce_count = bounces[i];
+            for (int b = 0; b < bounces[i] && b < K; ++b) {
+                const size_t at = i * K + b;
+                path.points.push_back({ hit_xyz[3 * at], hit_xyz[3 * at + 1], hit_xyz[3 * at + 2] });
+                path.hit_objects.push_back(patches_[hit_patch[at]].get());
+            }
+            switch (status[i]) {
+                case BBRT_ESCAPED: {
+                    path.status = RayPath::Status::Escaped;
+                    const double* f = &final_od[6 * i];   // RayTracingDriver.hpp:198-203
+                    path.points.push_back({ f[0] + f[3] * diag_, f[1] + f[4] * diag_, f[2] + f[5] * diag_ });
+                    break;
+                }
+                case BBRT_TRAPPED: path.status = RayPath::Status::Trapped; break;
+                default:
+                    path.status = RayPath::Status::Error;
+                    path.error_message = error_text(kind[i], err_patch[i]);
+            }
+        }
+        return out;
+    }
+
+    // RayTracingDriver.hpp:260-265
+    double self_intersect_epsilon(const ScatObject* leaving_object = nullptr) const {
+        const double padding = leaving_object ? leaving_object->self_intersect_padding() : max_patch_sagitta_;
+        return detail::greater(1e-9 * diag_, 5.0 * padding);
+    }
+
+    // RayTracingDriver.hpp:270-311, with 17 significant digits.
+    void write_vtk(const std::string& filename) const {
+        std::ofstream out(filename);
+        if (!out.is_open()) throw std::runtime_error("RayTracingDriver::write_vtk: cannot open output file: " + filename);
+        write_paths_vtk(out, paths_, "Billiard-ball ray traces", "RayID");
+    }
+
+    static void write_paths_vtk(std::ostream& out, const std::vector<RayPath>& paths, const char* title,
+                                const char* id_name) {
+        std::vector<Vec3> pts;
+        struct Seg { int a, b, id, k; };
+        std::vector<Seg> segs;
+        for (size_t r = 0; r < paths.size(); ++r) {
+            const int base = (int)pts.size();
+            for (const Vec3& p : paths[r].points) pts.push_back(p);
+            for (size_t k = 0; k + 1 < paths[r].points.size(); ++k)
+                segs.push_back({ base + (int)k, base + (int)k + 1, (int)r, (int)k });
+        }
+        out << std::setprecision(17);
+        out << "# vtk DataFile Version 3.0\n" << title << "\nASCII\nDATASET POLYDATA\n";
+        out << "POINTS " << pts.size() << " double\n";
+        for (const Vec3& p : pts) out << p.x << ' ' << p.y << ' ' << p.z << '\n';
+        out << "LINES " << segs.size() << ' ' << segs.size() * 3 << '\n';
+        for (const Seg& s : segs) out << "2 " << s.a << ' ' << s.b << '\n';
+        out << "CELL_DATA " << segs.size() << '\n';
+        out << "SCALARS " << id_name << " int 1\nLOOKUP_TABLE default\n";
+        for (const Seg& s : segs) out << s.id << '\n';
+        out << "SCALARS BounceIndex int 1\nLOOKUP_TABLE default\n";
+        for (const Seg& s : segs) out << s.k << '\n';
+    }
+
+    double scene_diagonal() const { return diag_; }
+    bbrt_scene scene() const { return scene_; }
+
+private:
+    std::vector<std::unique_ptr<SurfacePatch>> patches_;
+    std::unordered_map<const ScatObject*, size_t> index_by_ptr_;
+    std::vector<std::string> paths_loaded_;
+    std::vector<Vec3> sources_, directions_;
+    std::vector<RayPath> paths_;
+    bbrt_scene scene_ = nullptr;
+    double diag_ = 0.0, max_patch_sagitta_ = 0.0;
+
+  public:
+    std::string error_text(int kind, int patch) const {
+        const std::string zone = (patch >= 0 && patch < (int)patches_.size())
+                                     ? std::to_string(patches_[patch]->data().zoneID) : std::string("?");
+        switch (kind) {   // the throw sites' wording (SurfacePatch.hpp:618-646, RayTracingDriver.hpp:176-226)
+            case BBRT_ERR_SINGULAR:
+                return "SurfacePatch::intersect: singular Jacobian during Newton refinement (zoneID " + zone +
+                       "), likely a grazing ray";
+            case BBRT_ERR_NOCONV:
+                return "SurfacePatch::intersect: Newton refinement failed to converge in 20 iterations (zoneID " +
+                       zone + ")";
+            case BBRT_ERR_DOMAIN:
+                return "SurfacePatch::intersect: Newton refinement converged outside the patch domain (zoneID " +
+                       zone + ")";
+            case BBRT_ERR_BACKFACE:
+                return "RayTracingDriver::trace_ray: hit back face of patch (dot(dir, normal) > 0) — ray likely "
+                       "tunneled through a triangulation gap";
+            case BBRT_ERR_ZERO_DIR: return "RayTracingDriver::trace_ray: zero-length direction";
+            default: return "RayTracingDriver::trace_ray: BVH traversal stack overflow";
+        }
+    }
+  private:
+
+    // RayTracingDriver.hpp:346-373: every *.txt in the directory, sorted by path.
+    void load_patches(const std::string& patch_directory) {
+        namespace fs = std::filesystem;
+        if (!fs::exists(patch_directory) || !fs::is_directory(patch_directory))
+            throw std::runtime_error("RayTracingDriver: not a directory: " + patch_directory);
+        std::vector<fs::path> files;
+        for (const auto& e : fs::directory_iterator(patch_directory))
+            if (e.is_regular_file() && e.path().extension() == ".txt") files.push_back(e.path());
+        std::sort(files.begin(), files.end());
+        for (const auto& p : files) {
+            patches_.push_back(std::make_unique<SurfacePatch>(p.string()));
+            paths_loaded_.push_back(p.string());
+        }
+        if (patches_.empty())
+            throw std::runtime_error("RayTracingDriver: no *.txt patch files found in: " + patch_directory);
+    }
+
+    // RayTracingDriver.hpp:410-424
+    void load_patches_range(const std::string& directory, const std::string& prefix, int first, int last) {
+        if (last < first)
+            throw std::invalid_argument("RayTracingDriver: empty patch index range [" + std::to_string(first) + ", " +
+                                        std::to_string(last) + "]");
+        for (int i = first; i <= last; ++i) {
+            paths_loaded_.push_back(directory + prefix + std::to_string(i) + ".txt");
+            patches_.push_back(std::make_unique<SurfacePatch>(paths_loaded_.back()));
+        }
+    }
+
+    void build_scene() {
+        const size_t n = patches_.size();
+        std::vector<int32_t> imax(n), jmax(n);
+        std::vector<int64_t> uo(n + 1, 0), vo(n + 1, 0), fo(n + 1, 0);
+        for (size_t k = 0; k < n; ++k) {
+            const auto& d = patches_[k]->data();
+            if (d.kmax != 1) throw std::runtime_error("RayTracingDriver: only surface patches (kmax == 1) are supported");
+            imax[k] = d.imax; jmax[k] = d.jmax;
+            uo[k + 1] = uo[k] + d.imax; vo[k + 1] = vo[k] + d.jmax; fo[k + 1] = fo[k] + (int64_t)d.imax * d.jmax;
+        }
+        std::vector<double> un, vn, fields((size_t)12 * fo[n]);
+        for (size_t k = 0; k < n; ++k) {
+            const auto& d = patches_[k]->data();
+            un.insert(un.end(), d.uNodes.begin(), d.uNodes.end());
+            vn.insert(vn.end(), d.vNodes.begin(), d.vNodes.end());
+            const std::vector<double>* src[12] = { &d.x, &d.y, &d.z, &d.dxdu, &d.dydu, &d.dzdu,
+                                                   &d.dxdv, &d.dydv, &d.dzdv, &d.nuX, &d.nuY, &d.nuZ };
+            const size_t nn = (size_t)d.imax * d.jmax;
+            for (int f = 0; f < 12; ++f)
+                std::copy(src[f]->begin(), src[f]->begin() + nn, fields.begin() + 12 * fo[k] + f * nn);
+        }
+        bbrt_patch_set ps{ (int64_t)n, imax.data(), jmax.data(), uo.data(), vo.data(), fo.data(),
+                           un.data(), vn.data(), fields.data() };
+        detail::check(bbrt_scene_create(&ps, &scene_), "bbrt_scene_create");
+        std::vector<double> box(6 * n), cent(3 * n), ms(n);
+        detail::check(bbrt_scene_patch_info(scene_, box.data(), cent.data(), ms.data()), "bbrt_scene_patch_info");
+        for (size_t k = 0; k < n; ++k) {
+            patches_[k]->set_prepared(&box[6 * k], ms[k]);
+            index_by_ptr_[patches_[k].get()] = k;
+        }
+        detail::check(bbrt_scene_info(scene_, &diag_, &max_patch_sagitta_, nullptr, nullptr, nullptr),
+                      "bbrt_scene_info");
+    }
+};
+
+// --- PlaneSourceDriver (PlaneSourceDriver.hpp:55-244) -----------------------
+class PlaneSourceDriver {
+public:
+    explicit PlaneSourceDriver(const std::string& patch_directory) : driver_(patch_directory) {}
+
+    // PlaneSourceDriver.hpp:79-109; the per-patch counts accumulate on the device.
+    void run(const Vec3& corner0, const Vec3& corner1, const Vec3& direction, int nx, int ny, int max_bounces = 50,
+             bool count_direct_hit = false) {
+        if (nx < 1 || ny < 1) throw std::invalid_argument("PlaneSourceDriver::run: nx and ny must both be >= 1");
+        if (direction.x * direction.x + direction.y * direction.y + direction.z * direction.z < 1e-300)
+            throw std::invalid_argument("PlaneSourceDriver::run: direction has zero length");
+        const std::vector<Vec3> sources = build_plane_grid(corner0, corner1, nx, ny);
+        hit_count_.assign(driver_.patches().size(), 0);
+        const std::vector<Vec3> dirs(sources.size(), direction);
+        paths_ = driver_.trace_batch(sources, dirs, max_bounces, reinterpret_cast<int64_t*>(hit_count_.data()),
+                                     count_direct_hit);
+    }
+
+    // PlaneSourceDriver.hpp:115-178 (17 significant digits)
+    void write_vtk(const std::string& filename) const {
+        if (hit_count_.empty()) throw std::runtime_error("PlaneSourceDriver::write_vtk: no data — call run() first");
+        std::ofstream out(filename);
+        if (!out.is_open()) throw std::runtime_error("PlaneSourceDriver::write_vtk: cannot open output file: " + filename);
+        const auto& patches = driver_.patches();
+        std::vector<Vec3> pts;
+        std::vector<long long> hits;
+        struct Tri { int a, b, c; };
+        std::vector<Tri> tris;
+        for (size_t p = 0; p < patches.size(); ++p) {
+            const auto& d = patches[p]->data();
+            const int base = (int)pts.size();
+            for (int idx = 0; idx < d.imax * d.jmax; ++idx) { pts.push_back({ d.x[idx], d.y[idx], d.z[idx] }); hits.push_back(hit_count_[p]); }
+            for (int i = 0; i + 1 < d.imax; ++i)
+                for (int j = 0; j + 1 < d.jmax; ++j) {
+                    const int p00 = base + i * d.jmax + j, p10 = base + (i + 1) * d.jmax + j;
+                    const int p01 = base + i * d.jmax + j + 1, p11 = base + (i + 1) * d.jmax + j + 1;
+                    tris.push_back({ p00, p10, p11 });
+                    tris.push_back({ p00, p11, p01 });
+                }
+        }
+        out << std::setprecision(17);
+        out << "# vtk DataFile Version 3.0\nPlane-source patch hit counts\nASCII\nDATASET POLYDATA\n";
+        out << "POINTS " << pts.size() << " double\n";
+        for (const Vec3& p : pts) out << p.x << ' ' << p.y << ' ' << p.z << '\n';
+        out << "POLYGONS " << tris.size() << ' ' << tris.size() * 4 << '\n';
+        for (const Tri& t : tris) out << "3 " << t.a << ' ' << t.b << ' ' << t.c << '\n';
+        out << "POINT_DATA " << pts.size() << '\n' << "SCALARS HitCount long 1\nLOOKUP_TABLE default\n";
+        for (long long h : hits) out << h << '\n';
+    }
+
+    const RayTracingDriver& driver() const { return driver_; }
+    RayTracingDriver& driver() { return driver_; }
+    const std::vector<long long>& hit_counts() const { return hit_count_; }
+    const std::vector<RayTracingDriver::RayPath>& paths() const { return paths_; }
+
+    // PlaneSourceDriver.hpp:190-243
+    static std::vector<Vec3> build_plane_grid(const Vec3& corner0, const Vec3& corner1, int nx, int ny) {
+        const double c0[3] = { corner0.x, corner0.y, corner0.z }, c1[3] = { corner1.x, corner1.y, corner1.z };
+        const double scale = std::max({ 1.0, std::fabs(c0[0]), std::fabs(c0[1]), std::fabs(c0[2]), std::fabs(c1[0]),
+                                        std::fabs(c1[1]), std::fabs(c1[2]) });
+        const double tol = 1e-9 * scale;
+        int constant_axis = -1, match_count = 0;
+        for (int a = 0; a < 3; ++a)
+            if (std::fabs(c0[a] - c1[a]) < tol) { constant_axis = a; ++match_count; }
+        if (match_count != 1)
+            throw std::invalid_argument(
+                "PlaneSourceDriver: corner0 and corner1 must agree in exactly one coordinate to define an axis-aligned "
+                "plane (got " + std::to_string(match_count) + " matching coordinates)");
+        int axis_a = -1, axis_b = -1;
+        for (int a = 0; a < 3; ++a) {
+            if (a == constant_axis) continue;
+            if (axis_a < 0) axis_a = a; else axis_b = a;
+        }
+        std::vector<double> ga(nx), gb(ny);
+        for (int i = 0; i < nx; ++i)
+            ga[i] = (nx == 1) ? c0[axis_a] : c0[axis_a] + (c1[axis_a] - c0[axis_a]) * static_cast<double>(i) / (nx - 1);
+        for (int j = 0; j < ny; ++j)
+            gb[j] = (ny == 1) ? c0[axis_b] : c0[axis_b] + (c1[axis_b] - c0[axis_b]) * static_cast<double>(j) / (ny - 1);
+        std::vector<Vec3> pts;
+        pts.reserve((size_t)nx * ny);
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                double c[3];
+                c[constant_axis] = c0[constant_axis];
+                c[axis_a] = ga[i];
+                c[axis_b] = gb[j];
+                pts.push_back({ c[0], c[1], c[2] });
+            }
+        return pts;
+    }
+
+private:
+    RayTracingDriver driver_;
+    std::vector<long long> hit_count_;
+    std::vector<RayTracingDriver::RayPath> paths_;
+};
+
+// --- BounceMapDriver (BounceMapDriver.hpp:61-520) ----------------------------
+class BounceMapDriver {
+public:
+    BounceMapDriver(const std::string& patch_directory, const std::string& file_prefix, int first_index_1based,
+                    int last_index_1based)
+        : driver_(patch_directory), first_index_1based_(first_index_1based) {
+        if (last_index_1based < first_index_1based)
+            throw std::invalid_argument("BounceMapDriver: empty patch index range [" +
+                                        std::to_string(first_index_1based) + ", " + std::to_string(last_index_1based) +
+                                        "]");
+        for (int i = first_index_1based; i <= last_index_1based; ++i) {
+            const std::string path = patch_directory + file_prefix + std::to_string(i) + ".txt";
+            auto sp = std::make_unique<SurfacePatch>(path);
+            // Its launch nudge uses its own max sagitta (BounceMapDriver.hpp:161): the
+            // scene's copy of the same file was prepared with exactly that value.
+            const long long k = driver_.patch_index_of_path(path);
+            if (k < 0) throw std::runtime_error("BounceMapDriver: sampled patch is not in the scene directory: " + path);
+            const SurfacePatch& twin = *driver_.patches()[k];
+            const AABB& b = twin.bounding_box();
+            const double box[6] = { b.min.x, b.min.y, b.min.z, b.max.x, b.max.y, b.max.z };
+            sp->set_prepared(box, twin.max_sagitta());
+            sample_patches_.push_back(std::move(sp));
+        }
+    }
+
+    // BounceMapDriver.hpp:103-259: the launch rays of all sampled points go to
+    // the GPU in batches; per point the largest bounce count over directions
+    // that did not error and the error count are reduced on the host.  With
+    // capture_best_rays, the best direction of each point (first found on
+    // ties, :228-235) is traced once more with its full path recorded.
+    void run(int position_grid_n, int theta_n, int phi_n, double cone_angle_deg = 180.0, int max_bounces = 50,
+             bool capture_best_rays = false) {
+        if (position_grid_n < 2) throw std::invalid_argument("BounceMapDriver::run: position_grid_n must be >= 2");
+        if (theta_n < 1 || phi_n < 1) throw std::invalid_argument("BounceMapDriver::run: theta_n and phi_n must both be >= 1");
+        if (!(cone_angle_deg > 0.0) || cone_angle_deg > 180.0)
+            throw std::invalid_argument("BounceMapDriver::run: cone_angle_deg must be in (0, 180]");
+        launches_ = generate_launches(position_grid_n, theta_n, phi_n, cone_angle_deg);
+        capture_best_rays_ = capture_best_rays;
+        const size_t per_point = (size_t)theta_n * phi_n;
+        const size_t n_patch = sample_patches_.size();
+        const size_t pts_per_patch = (size_t)position_grid_n * position_grid_n;
+        const size_t n_rays = launches_.origins.size();
+        std::vector<int8_t> status(n_rays), kind(n_rays);
+        std::vector<int32_t> bounces(n_rays), err_patch(n_rays);
+        const size_t chunk = (size_t)1 << 23;
+        for (size_t a = 0; a < n_rays; a += chunk) {
+            const size_t m = std::min(chunk, n_rays - a);
+            std::vector<double> o(3 * m), d(3 * m);
+            for (size_t i = 0; i < m; ++i) {
+                const Vec3& oo = launches_.origins[a + i];
+                const Vec3& dd = launches_.dirs[a + i];
+                o[3 * i] = oo.x; o[3 * i + 1] = oo.y; o[3 * i + 2] = oo.z;
+                d[3 * i] = dd.x; d[3 * i + 1] = dd.y; d[3 * i + 2] = dd.z;
+            }
+            std::vector<int8_t> st, kd;
+            std::vector<int32_t> bc, ep;
+            driver_.trace_raw(o, d, max_bounces, 0, st, kd, bc, ep, nullptr, nullptr, nullptr);
+            std::copy(st.begin(), st.end(), status.begin() + a);
+            std::copy(bc.begin(), bc.end(), bounces.begin() + a);
+        }
+        patch_max_bounces_.assign(n_patch, std::vector<int>(pts_per_patch, 0));
+        patch_error_counts_.assign(n_patch, std::vector<int>(pts_per_patch, 0));
+        std::vector<size_t> best_ray(n_patch * pts_per_patch, SIZE_MAX);
+        for (size_t p = 0; p < n_patch; ++p)
+            for (size_t k = 0; k < pts_per_patch; ++k) {
+                int best = 0, n_errors = 0;
+                size_t arg = SIZE_MAX;
+                for (size_t r = 0; r < per_point; ++r) {
+                    const size_t ray = (p * pts_per_patch + k) * per_point + r;
+                    if (status[ray] == BBRT_ERROR) { ++n_errors; continue; }
+                    if (arg == SIZE_MAX || bounces[ray] > bounces[arg]) arg = ray;
+                    best = std::max(best, (int)bounces[ray]);
+                }
+                patch_max_bounces_[p][k] = best;
+                patch_error_counts_[p][k] = n_errors;
+                best_ray[p * pts_per_patch + k] = arg;
+            }
+        patch_best_paths_.clear();
+        if (capture_best_rays_) {
+            std::vector<Vec3> bo, bd;
+            std::vector<size_t> where;
+            for (size_t q = 0; q < best_ray.size(); ++q)
+                if (best_ray[q] != SIZE_MAX) {
+                    bo.push_back(launches_.origins[best_ray[q]]);
+                    bd.push_back(launches_.dirs[best_ray[q]]);
+                    where.push_back(q);
+                }
+            const auto paths = driver_.trace_batch(bo, bd, max_bounces);
+            patch_best_paths_.assign(n_patch, std::vector<RayTracingDriver::RayPath>(pts_per_patch));
+            for (size_t i = 0; i < where.size(); ++i)
+                patch_best_paths_[where[i] / pts_per_patch][where[i] % pts_per_patch] = paths[i];
+        }
+    }
+
+    struct Launches {
+        std::vector<Vec3> origins, dirs;
+        std::vector<SurfacePatch::SampledGrid> grids;
+    };
+
+    // BounceMapDriver.hpp:115-220: per sampled patch an n x n grid of surface
+    // points, each nudged off the surface along its normal, times a
+    // theta_n x phi_n solid-angle-uniform cone about the normal.
+    Launches generate_launches(int position_grid_n, int theta_n, int phi_n, double cone_angle_deg) const {
+        Launches L;
+        const double half_angle_deg = cone_angle_deg / 2.0;
+        const double cos_theta_max = std::cos(half_angle_deg * kTwoPi / 360.0);
+        for (size_t p = 0; p < sample_patches_.size(); ++p) {
+            const auto& patch = *sample_patches_[p];
+            const double eps = driver_.self_intersect_epsilon(sample_patches_[p].get());
+            SurfacePatch::SampledGrid grid = patch.sample_grid(position_grid_n);
+            const size_t nn = grid.x.size();
+            for (size_t k = 0; k < nn; ++k) {
+                const Vec3 point = { grid.x[k], grid.y[k], grid.z[k] };
+                const Vec3 normal = vec_normalize({ grid.nx[k], grid.ny[k], grid.nz[k] });
+                const Vec3 du = { grid.dxdu[k], grid.dydu[k], grid.dzdu[k] };
+                Vec3 t1, t2;
+                build_tangent_frame(normal, du, t1, t2);
+                const Vec3 origin = { point.x + normal.x * eps, point.y + normal.y * eps, point.z + normal.z * eps };
+                for (int ti = 0; ti < theta_n; ++ti) {
+                    const double mu = cos_theta_max + (static_cast<double>(ti) + 0.5) / static_cast<double>(theta_n) *
+                                                          (1.0 - cos_theta_max);
+                    const double sin_theta = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+                    for (int pj = 0; pj < phi_n; ++pj) {
+                        const double phi = (static_cast<double>(pj) + 0.5) * kTwoPi / static_cast<double>(phi_n);
+                        const double cos_phi = std::cos(phi), sin_phi = std::sin(phi);
+                        L.origins.push_back(origin);
+                        L.dirs.push_back({ t1.x * sin_theta * cos_phi + t2.x * sin_theta * sin_phi + normal.x * mu,
+                                           t1.y * sin_theta * cos_phi + t2.y * sin_theta * sin_phi + normal.y * mu,
+                                           t1.z * sin_theta * cos_phi + t2.z * sin_theta * sin_phi + normal.z * mu });
+                    }
+                }
+            }
+            L.grids.push_back(std::move(grid));
+        }
+        return L;
+    }
+
+    const std::vector<std::vector<int>>& max_bounces() const { return patch_max_bounces_; }
+    const std::vector<std::vector<int>>& error_counts() const { return patch_error_counts_; }
+    const std::vector<std::vector<RayTracingDriver::RayPath>>& best_paths() const { return patch_best_paths_; }
+    const Launches& launches() const { return launches_; }
+
+    // BounceMapDriver.hpp:267-335 (17 significant digits)
+    void write_vtk(const std::string& filename) const {
+        if (launches_.grids.empty()) throw std::runtime_error("BounceMapDriver::write_vtk: no data — call run() first");
+        std::ofstream out(filename);
+        if (!out.is_open()) throw std::runtime_error("BounceMapDriver::write_vtk: cannot open output file: " + filename);
+        std::vector<Vec3> pts;
+        std::vector<int> mb, ec;
+        struct Tri { int a, b, c; };
+        std::vector<Tri> tris;
+        for (size_t p = 0; p < launches_.grids.size(); ++p) {
+            const auto& g = launches_.grids[p];
+            const int n = g.n, base = (int)pts.size();
+            for (int idx = 0; idx < n * n; ++idx) {
+                pts.push_back({ g.x[idx], g.y[idx], g.z[idx] });
+                mb.push_back(patch_max_bounces_[p][idx]);
+                ec.push_back(patch_error_counts_[p][idx]);
+            }
+            for (int i = 0; i + 1 < n; ++i)
+                for (int j = 0; j + 1 < n; ++j) {
+                    const int p00 = base + i * n + j, p10 = base + (i + 1) * n + j, p01 = base + i * n + j + 1,
+                              p11 = base + (i + 1) * n + j + 1;
+                    tris.push_back({ p00, p10, p11 });
+                    tris.push_back({ p00, p11, p01 });
+                }
+        }
+        out << std::setprecision(17);
+        out << "# vtk DataFile Version 3.0\nBilliard-ball bounce map\nASCII\nDATASET POLYDATA\n";
+        out << "POINTS " << pts.size() << " double\n";
+        for (const Vec3& q : pts) out << q.x << ' ' << q.y << ' ' << q.z << '\n';
+        out << "POLYGONS " << tris.size() << ' ' << tris.size() * 4 << '\n';
+        for (const Tri& t : tris) out << "3 " << t.a << ' ' << t.b << ' ' << t.c << '\n';
+        out << "POINT_DATA " << pts.size() << '\n' << "SCALARS MaxBounces int 1\nLOOKUP_TABLE default\n";
+        for (int b : mb) out << b << '\n';
+        out << "SCALARS ErrorCount int 1\nLOOKUP_TABLE default\n";
+        for (int e : ec) out << e << '\n';
+    }
+
+    // BounceMapDriver.hpp:346-408
+    void write_best_ray_vtks(const std::string& output_directory,
+                             const std::string& file_prefix = "best_rays_patch_") const {
+        if (!capture_best_rays_)
+            throw std::runtime_error("BounceMapDriver::write_best_ray_vtks: best rays weren't captured — call "
+                                     "run(..., /*capture_best_rays=*/true) first");
+        if (patch_best_paths_.empty())
+            throw std::runtime_error("BounceMapDriver::write_best_ray_vtks: no data — call run() first");
+        std::filesystem::create_directories(output_directory);
+        for (size_t p = 0; p < patch_best_paths_.size(); ++p) {
+            const std::string fn = output_directory + "/" + file_prefix +
+                                   std::to_string(first_index_1based_ + (int)p) + ".vtk";
+            std::ofstream out(fn);
+            if (!out.is_open())
+                throw std::runtime_error("BounceMapDriver::write_best_ray_vtks: cannot open output file: " + fn);
+            RayTracingDriver::write_paths_vtk(out, patch_best_paths_[p], "Best bounce ray per grid point", "PointID");
+        }
+    }
+
+    const RayTracingDriver& driver() const { return driver_; }
+
+    // BounceMapDriver.hpp:499-519
+    static void build_tangent_frame(const Vec3& n, const Vec3& du, Vec3& t1, Vec3& t2) {
+        const double du_len2 = vec_dot(du, du);
+        const double du_dot_n = vec_dot(du, n);
+        const Vec3 du_perp = { du.x - du_dot_n * n.x, du.y - du_dot_n * n.y, du.z - du_dot_n * n.z };
+        const double du_perp_len2 = vec_dot(du_perp, du_perp);
+        if (du_perp_len2 > kTangentDegenerateRelTol * kTangentDegenerateRelTol * du_len2) {
+            t1 = vec_normalize(du_perp);
+        } else {
+            const Vec3 a = (std::fabs(n.x) <= std::fabs(n.y) && std::fabs(n.x) <= std::fabs(n.z))
+                               ? Vec3{ 1.0, 0.0, 0.0 }
+                               : (std::fabs(n.y) <= std::fabs(n.z) ? Vec3{ 0.0, 1.0, 0.0 } : Vec3{ 0.0, 0.0, 1.0 });
+            t1 = vec_normalize(vec_cross(n, a));
+        }
+        t2 = vec_cross(n, t1);
+    }
+
+private:
+    RayTracingDriver driver_;
+    std::vector<std::unique_ptr<SurfacePatch>> sample_patches_;
+    int first_index_1based_ = 1;
+    bool capture_best_rays_ = false;
+    Launches launches_;
+    std::vector<std::vector<int>> patch_max_bounces_, patch_error_counts_;
+    std::vector<std::vector<RayTracingDriver::RayPath>> patch_best_paths_;
+    static constexpr double kTwoPi = 6.283185307179586476925286766559;
+    static constexpr double kTangentDegenerateRelTol = 1e-6;
+
+};
+
+}  // namespace bbrt_cpp

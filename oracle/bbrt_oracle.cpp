@@ -517,7 +517,12 @@ void trace_ray(const Scene& S, int64_t ray, const Vec3& origin, const Vec3& dire
     double* fod = out.final_od + ray * 6;
     for (int c = 0; c < 6; ++c) fod[c] = 0.0;
     const double len2 = direction.x * direction.x + direction.y * direction.y + direction.z * direction.z;
-    if (len2 < 1e-300) { out.status[ray] = kError; out.err_kind[ray] = kErrZeroDir; return; }
+    if (len2 < 1e-300) {
+        out.status[ray] = kError; out.err_kind[ray] = kErrZeroDir;
+        fod[0] = origin.x; fod[1] = origin.y; fod[2] = origin.z;
+        fod[3] = direction.x; fod[4] = direction.y; fod[5] = direction.z;
+        return;
+    }
     Vec3 o = origin;
     const double len = std::sqrt(direction.x * direction.x + direction.y * direction.y + direction.z * direction.z);
     Vec3 d = { direction.x / len, direction.y / len, direction.z / len };
@@ -791,5 +796,66 @@ void orc_trace(void* s, const double* origins, const double* dirs, int64_t n, in
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
+
+// BounceMapDriver::run's launch generation (BounceMapDriver.hpp:115-220) for the
+// scene patches `ids` in order: SurfacePatch::sample_grid (SurfacePatch.hpp:199-242),
+// build_tangent_frame (BounceMapDriver.hpp:499-519), the launch nudge along the
+// normal by self_intersect_epsilon(patch) (RayTracingDriver.hpp:260-265) and the
+// theta x phi cone.  Writes [n_ids][n*n][theta_n*phi_n][3] origins and directions.
+void orc_bounce_map_launches(void* s, const int64_t* ids, int64_t n_ids, int n, int theta_n, int phi_n,
+                             double cone_deg, double* origins, double* dirs) {
+    const Scene& S = *static_cast<Scene*>(s);
+    constexpr double kTwoPi = 6.283185307179586476925286766559;
+    const double half = cone_deg / 2.0;
+    const double cos_theta_max = std::cos(half * kTwoPi / 360.0);
+    std::vector<double> g(n);
+    for (int i = 0; i < n; ++i) g[i] = (n == 1) ? 0.0 : -1.0 + 2.0 * static_cast<double>(i) / (n - 1);
+    size_t at = 0;
+    for (int64_t q = 0; q < n_ids; ++q) {
+        const Patch& P = S.patches[ids[q]];
+        const double eps = smax(kSelfEpsFactor * S.diag, kSagittaClearance * P.max_sag);
+        const size_t nn = (size_t)n * n;
+        std::vector<double> F[9];
+        const int fidx[9] = { 0, 1, 2, 9, 10, 11, 3, 4, 5 };   // x y z, nu, dX/du
+        for (int f = 0; f < 9; ++f) {
+            F[f].resize(nn);
+            interp2d(P.un.data(), P.imax, P.vn.data(), P.jmax, P.uw.data(), P.vw.data(), P.f[fidx[f]].data(), g.data(),
+                     n, g.data(), n, F[f].data());
+        }
+        for (size_t k = 0; k < nn; ++k) {
+            const Vec3 nrm0 = vnormalize({ F[3][k], F[4][k], F[5][k] });   // sample_grid renormalises
+            const Vec3 point = { F[0][k], F[1][k], F[2][k] };
+            const Vec3 normal = vnormalize(nrm0);                          // and run() normalises again
+            const Vec3 du = { F[6][k], F[7][k], F[8][k] };
+            const double du_len2 = vdot(du, du);
+            const double du_dot_n = vdot(du, normal);
+            const Vec3 du_perp = { du.x - du_dot_n * normal.x, du.y - du_dot_n * normal.y, du.z - du_dot_n * normal.z };
+            const double du_perp_len2 = vdot(du_perp, du_perp);
+            Vec3 t1;
+            if (du_perp_len2 > 1e-6 * 1e-6 * du_len2) t1 = vnormalize(du_perp);
+            else {
+                const Vec3 a = (std::fabs(normal.x) <= std::fabs(normal.y) && std::fabs(normal.x) <= std::fabs(normal.z))
+                                   ? Vec3{ 1.0, 0.0, 0.0 }
+                                   : (std::fabs(normal.y) <= std::fabs(normal.z) ? Vec3{ 0.0, 1.0, 0.0 } : Vec3{ 0.0, 0.0, 1.0 });
+                t1 = vnormalize(vcross(normal, a));
+            }
+            const Vec3 t2 = vcross(normal, t1);
+            const Vec3 origin = { point.x + normal.x * eps, point.y + normal.y * eps, point.z + normal.z * eps };
+            for (int ti = 0; ti < theta_n; ++ti) {
+                const double mu = cos_theta_max + (static_cast<double>(ti) + 0.5) / static_cast<double>(theta_n) * (1.0 - cos_theta_max);
+                const double sin_theta = std::sqrt(smax(0.0, 1.0 - mu * mu));
+                for (int pj = 0; pj < phi_n; ++pj) {
+                    const double phi = (static_cast<double>(pj) + 0.5) * kTwoPi / static_cast<double>(phi_n);
+                    const double cp = std::cos(phi), sp = std::sin(phi);
+                    origins[3 * at] = origin.x; origins[3 * at + 1] = origin.y; origins[3 * at + 2] = origin.z;
+                    dirs[3 * at] = t1.x * sin_theta * cp + t2.x * sin_theta * sp + normal.x * mu;
+                    dirs[3 * at + 1] = t1.y * sin_theta * cp + t2.y * sin_theta * sp + normal.y * mu;
+                    dirs[3 * at + 2] = t1.z * sin_theta * cp + t2.z * sin_theta * sp + normal.z * mu;
+                    ++at;
+                }
+            }
+        }
+    }
+}
 
 }  // extern "C"

@@ -63,6 +63,7 @@ def lib():
         L.orc_trace.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, _i8, _i8, _i32,
                                 _i32, _i32, _d, _u8, _d, _i64]
         L.orc_max_threads.restype = C.c_int
+        L.orc_bounce_map_launches.restype = None
         _LIB = L
     return _LIB
 
@@ -238,6 +239,16 @@ class OracleScene:
                         C.c_int(K), C.c_int(threads), ptr(st, _i8), ptr(ek, _i8), ptr(b, _i32), ptr(ep, _i32),
                         ptr(hp, _i32), ptr(hx, _d), ptr(it, _u8), ptr(fod, _d), ptr(cnt, _i64))
         return TraceResult(st, ek, b, ep, hp, hx, it, fod, cnt)
+
+
+def bounce_map_launches(osc, ids, n, theta_n, phi_n, cone_deg):
+    """Launch rays of BounceMapDriver::run for scene patches `ids` (oracle restatement)."""
+    ids = np.ascontiguousarray(ids, dtype=np.int64)
+    m = ids.size * n * n * theta_n * phi_n
+    o = np.zeros((m, 3)); d = np.zeros((m, 3))
+    lib().orc_bounce_map_launches(C.c_void_p(osc.h), ptr(ids, _i64), C.c_int64(ids.size), C.c_int(n), C.c_int(theta_n),
+                                  C.c_int(phi_n), C.c_double(cone_deg), ptr(o, _d), ptr(d, _d))
+    return o, d
 
 
 def max_threads() -> int:

@@ -98,3 +98,42 @@ def test_culling_equals_brute_force(name):
     assert_same(a, b, name + " culled vs brute force")
     assert b.counter("mt_tests") == 6 * 99 * 99 * b.counter("candidates")
     assert a.counter("mt_tests") < b.counter("mt_tests")
+
+
+def test_edge_cases_match_oracle():
+    """Zero directions, rays starting inside a sphere (back face), axis-parallel
+    rays, max_bounces 0 and 1, a ragged ray count, records shorter than the
+    bounce cap, and an empty batch."""
+    from oracle import oracle as O
+    sc, o, d, mb = _scene("c1")
+    rng = np.random.default_rng(3)
+    n = 333
+    oo = rng.uniform(-3, 3, (n, 3))
+    dd = rng.normal(size=(n, 3))
+    dd[::11] = 0.0                                    # zero-length: Error, zero_dir
+    oo[1::13] = [-1.5, 0.1, 0.2]                      # inside the left sphere: back-face hits
+    dd[2::17] = [[1.0, 0.0, 0.0]]                     # axis-parallel: 0 * inf in the slab test
+    gs = _gpu_scene("c1", sc)
+    osc = O.OracleScene(sc)
+    for max_b, rec in [(0, 1), (1, 1), (20, 20), (20, 5)]:
+        print(f"[edge] max_bounces={max_b} record_k={rec}", flush=True)
+        g = gs.trace(oo, dd, max_b, mode=L.NEWTON_REF, record_k=rec)
+        r = osc.trace(oo, dd, max_b, mode=0, record_k=rec)
+        assert_same(g, r, f"edge max_b={max_b}")
+    assert (g.err_kind == L.ERR_ZERO_DIR).sum() == int(np.all(dd == 0.0, axis=1).sum()) > 0
+    assert (g.err_kind == L.ERR_BACKFACE).sum() > 0
+    e = gs.trace(np.zeros((0, 3)), np.zeros((0, 3)), 20)
+    assert e.status.size == 0 and e.counter("rays") == 0
+
+
+def test_hit_counts_on_device():
+    """PlaneSourceDriver semantics (PlaneSourceDriver.hpp:96-108) accumulated on the GPU."""
+    sc, o, d, mb = _scene("c1")
+    gs = _gpu_scene("c1", sc)
+    for direct in (False, True):
+        g = gs.trace(o, d, mb, mode=L.NEWTON_REF, hit_count=True, flags=L.FLAG_COUNT_DIRECT_HIT if direct else 0)
+        want = np.zeros(sc.n_patches, np.int64)
+        for i in range(o.shape[0]):
+            for k in range(0 if direct else 1, g.bounces[i]):
+                want[g.hit_patch[i, k]] += 1
+        np.testing.assert_array_equal(g.hit_count, want)
