@@ -127,8 +127,9 @@ def main():
                    "max_bounces": mb, "record_k": K, "newton_mode": args.mode, "seed_mode": "triangle",
                    "culling": not args.no_cull, "scene_seed": args.seed,
                    "schedule": "wavefront" if args.wavefront else "persistent", "block": args.block or 256,
-                   "waves_per_eu": args.waves_per_eu or 2},
-        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "waves_per_eu": args.waves_per_eu or 3},
+        "roofline": {"bound": "mfma", "compute": "fp64 (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
+                     "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_bounce" if args.wavefront else "k_persist", "launches_per_step": launches / args.steps,
                      "avg_launch_ms": kernel_ms / launches if launches else None,
@@ -139,9 +140,26 @@ def main():
         "scene_build_s": t_build,
     }
 
+    traffic = committed_traffic(args, n, mb, K)
+    if traffic is not None:
+        res["roofline"]["traffic"] = traffic
+        res["roofline"]["traffic_source"] = "profiles/k_persist_sinai_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)"
     if args.cpu_seconds > 0:
         res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, mode, args.cpu_seconds, gs, K)
     print(json.dumps(res), flush=True)
+
+
+def committed_traffic(args, n, mb, K):
+    """HBM bytes per k_persist launch measured by rocprofv3 for the default workload, or None."""
+    if (n, mb, K, args.mode, args.seed, args.no_cull, args.wavefront) != (1 << 20, 100, 100, "converged", 0, False, False):
+        return None
+    path = os.path.join(ROOT, "profiles", "k_persist_sinai_1M.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def copy_probe(mb):

@@ -43,7 +43,7 @@ namespace {
 thread_local std::string g_last_error;
 
 constexpr int kStackCap = 48;
-constexpr int kDefaultWavesPerEu = 2;         // persistent-kernel occupancy target (autotuned per scene)
+constexpr int kDefaultWavesPerEu = 3;         // persistent-kernel occupancy target (best of 2/3/4 on MI355X)
 constexpr double kMtEps = 1e-9;               // MollerTrumbore.hpp:33
 constexpr double kOnNodeTol = 1.0e-12;        // PatchInterpolation.hpp:21
 constexpr int kNewtonMaxIters = 20;           // SurfacePatch.hpp:121
