@@ -626,67 +626,86 @@ __device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32
 
 // One bounce of ray r: RayTracingDriver.hpp:186-237.  Returns true while the
 // ray continues; o and d are advanced in place.
-template <bool WORK>
-__device__ bool bounce_once(const DevScene& S, const Outs& O, int32_t r, V3& o, V3& d, int bounce, int max_bounces,
-                            int record_k, int mode, uint32_t flags, bool root_live, Counters& c) {
-    Counters c0;
-    if constexpr (WORK) c0 = c;
-    const bool finite = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(d.x) && isfinite(d.y) &&
-                        isfinite(d.z);
-    const bool cull = !(flags & BBRT_FLAG_NO_CULL) && finite && fabs(o.x) <= S.cull_limit &&
-                      fabs(o.y) <= S.cull_limit && fabs(o.z) <= S.cull_limit;
-    const Ray32 r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
-    PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
-    int32_t best_obj, err_obj;
-    const int err = traverse(S, o, d, r32, cull, mode, root_live, best, best_obj, err_obj, c);
-    bool alive = false;
+// The end of one bounce, RayTracingDriver.hpp:190-237: error, escape, back
+// face, or record + reflect + nudge.  Returns true while the ray continues;
+// o and d are advanced in place.
+__device__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, V3& o, V3& d, int bounce, int max_bounces,
+                              int record_k, uint32_t flags, int err, int32_t err_obj, const PHit& best,
+                              int32_t best_obj, Counters& c) {
     if (err) {
         if (O.status) O.status[r] = bbrt::kError;
         if (O.err_kind) O.err_kind[r] = (int8_t)err;
         if (O.err_patch) O.err_patch[r] = err_obj;
         if (O.bounces) O.bounces[r] = bounce;
         write_final(O, r, o, d);
-    } else if (best_obj < 0) {
+        return false;
+    }
+    if (best_obj < 0) {
         if (O.status) O.status[r] = bbrt::kEscaped;
         if (O.bounces) O.bounces[r] = bounce;
         write_final(O, r, o, d);
-    } else if (dot3(d, best.n) > kBackfaceTol) {
+        return false;
+    }
+    if (dot3(d, best.n) > kBackfaceTol) {
         if (O.status) O.status[r] = bbrt::kError;
         if (O.err_kind) O.err_kind[r] = bbrt::kErrBackface;
         if (O.err_patch) O.err_patch[r] = best_obj;
         if (O.bounces) O.bounces[r] = bounce;
         write_final(O, r, o, d);
-    } else {
-        if (bounce < record_k) {
-            const size_t at = (size_t)r * record_k + bounce;
-            if (O.hit_patch) O.hit_patch[at] = best_obj;
-            if (O.newton_iters) O.newton_iters[at] = (uint8_t)best.iters;
-            if (O.hit_xyz) { O.hit_xyz[3 * at] = best.p.x; O.hit_xyz[3 * at + 1] = best.p.y; O.hit_xyz[3 * at + 2] = best.p.z; }
-        }
-        if (O.hit_count && (bounce >= 1 || (flags & BBRT_FLAG_COUNT_DIRECT_HIT)))
-            atomicAdd(O.hit_count + best_obj, 1ull);
-        c.bounces++;
-        // reflect (RayTracingDriver.hpp:405-408) and nudge (:234-237, :260-265)
-        const V3& nn = best.n;
-        const double dn = d.x * nn.x + d.y * nn.y + d.z * nn.z;
-        d = { d.x - 2.0 * dn * nn.x, d.y - 2.0 * dn * nn.y, d.z - 2.0 * dn * nn.z };
-        const double eps = smax(kSelfEpsFactor * S.diag, kSagittaClearance * S.meta[best_obj].max_sag);
-        o = { best.p.x + d.x * eps, best.p.y + d.y * eps, best.p.z + d.z * eps };
-        if (bounce + 1 >= max_bounces) {
-            if (O.status) O.status[r] = bbrt::kTrapped;
-            if (O.bounces) O.bounces[r] = bounce + 1;
-            write_final(O, r, o, d);
-        } else {
-            alive = true;
-        }
+        return false;
     }
-    if constexpr (WORK) {
-        uint32_t* w = O.ray_work + 4 * (size_t)r;
-        w[0] += (uint32_t)(c.bvh_tests - c0.bvh_tests);
-        w[1] += (uint32_t)(c.cull_tests - c0.cull_tests);
-        w[2] += (uint32_t)(c.mt_tests - c0.mt_tests);
-        w[3] += (uint32_t)(c.newton_iters - c0.newton_iters);
+    if (bounce < record_k) {
+        const size_t at = (size_t)r * record_k + bounce;
+        if (O.hit_patch) O.hit_patch[at] = best_obj;
+        if (O.newton_iters) O.newton_iters[at] = (uint8_t)best.iters;
+        if (O.hit_xyz) { O.hit_xyz[3 * at] = best.p.x; O.hit_xyz[3 * at + 1] = best.p.y; O.hit_xyz[3 * at + 2] = best.p.z; }
     }
+    if (O.hit_count && (bounce >= 1 || (flags & BBRT_FLAG_COUNT_DIRECT_HIT)))
+        atomicAdd(O.hit_count + best_obj, 1ull);
+    c.bounces++;
+    // reflect (RayTracingDriver.hpp:405-408) and nudge (:234-237, :260-265)
+    const V3& nn = best.n;
+    const double dn = d.x * nn.x + d.y * nn.y + d.z * nn.z;
+    d = { d.x - 2.0 * dn * nn.x, d.y - 2.0 * dn * nn.y, d.z - 2.0 * dn * nn.z };
+    const double eps = smax(kSelfEpsFactor * S.diag, kSagittaClearance * S.meta[best_obj].max_sag);
+    o = { best.p.x + d.x * eps, best.p.y + d.y * eps, best.p.z + d.z * eps };
+    if (bounce + 1 >= max_bounces) {
+        if (O.status) O.status[r] = bbrt::kTrapped;
+        if (O.bounces) O.bounces[r] = bounce + 1;
+        write_final(O, r, o, d);
+        return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool cull_ok(const DevScene& S, const V3& o, const V3& d, uint32_t flags) {
+    const bool finite = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(d.x) && isfinite(d.y) &&
+                        isfinite(d.z);
+    return !(flags & BBRT_FLAG_NO_CULL) && finite && fabs(o.x) <= S.cull_limit && fabs(o.y) <= S.cull_limit &&
+           fabs(o.z) <= S.cull_limit;
+}
+
+__device__ __forceinline__ void add_ray_work(const Outs& O, int32_t r, const Counters& c, const Counters& c0) {
+    uint32_t* w = O.ray_work + 4 * (size_t)r;
+    w[0] += (uint32_t)(c.bvh_tests - c0.bvh_tests);
+    w[1] += (uint32_t)(c.cull_tests - c0.cull_tests);
+    w[2] += (uint32_t)(c.mt_tests - c0.mt_tests);
+    w[3] += (uint32_t)(c.newton_iters - c0.newton_iters);
+}
+
+// One whole bounce of ray r (the wavefront schedule).
+template <bool WORK>
+__device__ bool bounce_once(const DevScene& S, const Outs& O, int32_t r, V3& o, V3& d, int bounce, int max_bounces,
+                            int record_k, int mode, uint32_t flags, bool root_live, Counters& c) {
+    Counters c0;
+    if constexpr (WORK) c0 = c;
+    const bool cull = cull_ok(S, o, d, flags);
+    const Ray32 r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
+    PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+    int32_t best_obj, err_obj;
+    const int err = traverse(S, o, d, r32, cull, mode, root_live, best, best_obj, err_obj, c);
+    const bool alive = finish_bounce(S, O, r, o, d, bounce, max_bounces, record_k, flags, err, err_obj, best, best_obj, c);
+    if constexpr (WORK) add_ray_work(O, r, c, c0);
     return alive;
 }
 
@@ -744,22 +763,55 @@ __global__ void k_init(const double* __restrict__ origins, const double* __restr
     }
 }
 
-// Persistent bounce loop: lanes take rays from the live list through one
-// wave-aggregated atomic, run them bounce after bounce with the ray state in
-// registers, and take the next ray when one ends.  Every wave leaves once the
-// list is exhausted and its lanes' rays have ended (each ray ends within
-// max_bounces bounces).
+// Persistent bounce loop as a per-lane state machine under a wave-uniform
+// outer loop.  Each iteration: lanes without a ray take one from the live list
+// (one wave-aggregated atomic); traversing lanes walk internal nodes until they
+// stand on a leaf (while-while); lanes on a leaf intersect one of its objects;
+// lanes whose bounce is complete record it, reflect, and start the next
+// bounce at once -- a lane never waits for another lane's whole bounce.  Every
+// ray ends within max_bounces bounces and the list is finite, so every wave
+// leaves the loop.
+enum : int { kPhTrav = 1, kPhLeaf = 2, kPhDone = 3 };
+
 template <int W, bool WORK>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k_persist(DevScene S, RayState R, const int32_t* __restrict__ live,
-                                                 const int32_t* __restrict__ n_live_p, int32_t* head, Outs O,
-                                                 int max_bounces, int record_k, int mode, uint32_t flags,
-                                                 unsigned long long* ctr) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
+k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_t* __restrict__ n_live_p,
+          int32_t* head, Outs O, int max_bounces, int record_k, int mode, uint32_t flags, unsigned long long* ctr) {
     const int n_live = *n_live_p;
     const int lane = __lane_id();
     Counters c = {};
+    Counters c0 = {};
     int32_t r = -1;   // -1: wants a ray, -2: list exhausted
-    int b = 0;
+    int b = 0, phase = 0;
     V3 o = { 0, 0, 0 }, d = { 0, 0, 0 };
+    double inv[3] = { 0, 0, 0 };
+    bool cull = false;
+    Ray32 r32 = { 0, 0, 0, 0, 0, 0 };
+    Ent stack[kStackCap];
+    int sp = 0;
+    Ent cur = { -1, 0, 0.0 };
+    double best_t = INFINITY;
+    PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+    int32_t best_obj = -1, err_obj = -1;
+    int err = 0;
+    int leaf_start = 0, leaf_count = 0, q = 0;
+    uint32_t enter = 0;
+
+    // bounce set-up: RayTracingDriver.hpp:187 and the root of BVH.hpp:89
+    auto start_bounce = [&](bool root_live) {
+        cull = cull_ok(S, o, d, flags);
+        r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
+        inv[0] = 1.0 / d.x; inv[1] = 1.0 / d.y; inv[2] = 1.0 / d.z;
+        best_t = INFINITY;
+        best_obj = -1; err_obj = -1; err = 0; sp = 0;
+        const BvhNode root = S.bvh[0];
+        double tm = -INFINITY;
+        const bool ok = root_live ? true : box_static(o, inv, root, tm);
+        cur = make_ent(0, root, ok, tm);
+        phase = kPhTrav;
+        if constexpr (WORK) c0 = c;
+    };
+
     for (;;) {
         const unsigned long long want = __ballot(r == -1);
         if (want) {
@@ -768,21 +820,86 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) k
             if (lane == leader) base = atomicAdd(head, (int)__popcll(want));
             base = __shfl(base, leader);
             if (r == -1) {
-                const int q = base + (int)__popcll(want & ((1ull << lane) - 1ull));
-                if (q < n_live) {
-                    r = live[q];
+                const int qq = base + (int)__popcll(want & ((1ull << lane) - 1ull));
+                if (qq < n_live) {
+                    r = live[qq];
                     o = { R.o[3 * r], R.o[3 * r + 1], R.o[3 * r + 2] };
                     d = { R.d[3 * r], R.d[3 * r + 1], R.d[3 * r + 2] };
                     b = 0;
+                    start_bounce(true);   // k_init already tested the root box
                 } else {
                     r = -2;
                 }
             }
         }
         if (__ballot(r >= 0) == 0) break;
-        if (r >= 0) {
-            if (bounce_once<WORK>(S, O, r, o, d, b, max_bounces, record_k, mode, flags, b == 0, c)) ++b;
-            else r = -1;
+
+        // traversal: internal nodes until a leaf to intersect, or the end (BVH.hpp:91-126)
+        if (r >= 0 && phase == kPhTrav) {
+            for (;;) {
+                c.bvh_tests++;
+                const bool live_node = cur.idx >= 0 && !(cur.tmin > best_t);
+                if (live_node && cur.meta < 0) {
+                    const int v = -1 - cur.meta;
+                    leaf_start = v >> 3; leaf_count = v & 7; q = 0;
+                    // the conservative box of each object's culling pyramid, for the whole leaf at once
+                    enter = 0;
+                    if (cull) {
+                        for (int k = 0; k < leaf_count; ++k) {
+                            const int32_t obj = S.leaf[leaf_start + k];
+                            const float4* b4 = reinterpret_cast<const float4*>(S.box_patch + (size_t)obj * 8);
+                            const float4 a = b4[0], bb = b4[1];
+                            const float bx[6] = { a.x, a.y, a.z, a.w, bb.x, bb.y };
+                            if (box32(bx, r32)) enter |= 1u << k;
+                        }
+                        c.cull_tests += leaf_count;
+                    } else {
+                        enter = 0x7Fu;
+                    }
+                    phase = kPhLeaf;
+                    break;
+                }
+                if (!live_node) {
+                    if (sp == 0) { phase = kPhDone; break; }
+                    cur = stack[--sp];
+                    continue;
+                }
+                const int32_t left = cur.idx + 1, right = cur.meta >> 2;
+                const BvhNode nl = S.bvh[left];
+                const BvhNode nr = S.bvh[right];
+                double tl = 0.0, tr = 0.0;
+                const bool okl = box_static(o, inv, nl, tl);
+                const bool okr = box_static(o, inv, nr, tr);
+                if (sp + 1 > kStackCap) { err = bbrt::kErrStack; phase = kPhDone; break; }
+                const Ent el = make_ent(left, nl, okl, tl), er = make_ent(right, nr, okr, tr);
+                const bool negax = (cur.meta & 3) == 0 ? inv[0] < 0.0 : ((cur.meta & 3) == 1 ? inv[1] < 0.0 : inv[2] < 0.0);
+                if (negax) { stack[sp++] = el; cur = er; }
+                else { stack[sp++] = er; cur = el; }
+            }
+        }
+
+        // one leaf object: SurfacePatch::intersect and the best-hit update (BVH.hpp:101-109)
+        if (r >= 0 && phase == kPhLeaf) {
+            const int32_t obj = S.leaf[leaf_start + q];
+            if (enter & (1u << q)) {
+                const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
+                if (h.status < 0) { err = -h.status; err_obj = obj; phase = kPhDone; }
+                else if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
+            } else {
+                c.candidates++;
+            }
+            if (phase == kPhLeaf && ++q == leaf_count) {
+                if (sp == 0) phase = kPhDone;
+                else { cur = stack[--sp]; phase = kPhTrav; }
+            }
+        }
+
+        if (r >= 0 && phase == kPhDone) {
+            const bool alive = finish_bounce(S, O, r, o, d, b, max_bounces, record_k, flags, err, err_obj, best,
+                                             best_obj, c);
+            if constexpr (WORK) add_ray_work(O, r, c, c0);
+            if (alive) { ++b; start_bounce(false); }
+            else { r = -1; phase = 0; }
         }
     }
     flush_counters(c, ctr);
