@@ -1,0 +1,95 @@
+// runtime.h -- host-side runtime shared by the C ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+
+namespace bbrt {
+// The thread-local message behind bbrt_last_error().
+extern thread_local std::string g_last_error;
+}  // namespace bbrt
+using bbrt::g_last_error;
+
+namespace {
+// ---------------------------------------------------------------------------
+// Host runtime
+// ---------------------------------------------------------------------------
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* dev_upload(const std::vector<T>& v, std::vector<void*>& allocs, int64_t& bytes) {
+    T* p = nullptr;
+    const size_t sz = std::max<size_t>(1, v.size()) * sizeof(T);
+    check(hipMalloc(&p, sz), "hipMalloc(scene)");
+    allocs.push_back(p);
+    bytes += (int64_t)sz;
+    if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(scene)");
+    return p;
+}
+
+}  // namespace
+
+
+struct bbrt_scene_s {
+    int device = 0;
+    int n_cu = 1;
+    hipStream_t stream = nullptr;
+    DevScene dev{};
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    int64_t n_patches = 0;
+    // host copies for the query calls
+    std::vector<double> pbox, pcent, max_sag;
+    std::vector<BvhNode> bvh;
+    std::vector<int32_t> leaf;
+    double diag = 0.0, max_patch_sag = 0.0;
+    // per-trace scratch, grown on demand
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int32_t* h_count = nullptr;   // pinned
+    hipEvent_t ev[2] = { nullptr, nullptr };
+    std::vector<hipEvent_t> kev;
+
+    ~bbrt_scene_s() {
+        for (void* p : allocs) (void)hipFree(p);
+        if (scratch) (void)hipFree(scratch);
+        if (h_count) (void)hipHostFree(h_count);
+        for (auto e : ev) if (e) (void)hipEventDestroy(e);
+        for (auto e : kev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+
+int fail(const std::exception& e) {
+    g_last_error = e.what();
+    return -1;
+}
+
+void* scratch_for(bbrt_scene s, size_t need) {
+    if (need > s->scratch_bytes) {
+        if (s->scratch) check(hipFree(s->scratch), "hipFree(scratch)");
+        s->scratch = nullptr;
+        s->scratch_bytes = 0;
+        check(hipMalloc(&s->scratch, need), "hipMalloc(scratch)");
+        s->scratch_bytes = need;
+    }
+    return s->scratch;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+}  // namespace
