@@ -40,6 +40,12 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-eu", type=int, default=0)
     ap.add_argument("--wavefront", action="store_true", help="per-bounce launches instead of the persistent kernel")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed leg")
+    ap.add_argument("--train-samples", type=int, default=20_000_000)
+    ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--train-batch", type=int, default=8192)
+    ap.add_argument("--train-lr", type=float, default=3e-3)
+    ap.add_argument("--newton-batch", type=int, default=0, help="lanes that run Newton together in the persistent kernel (0 = library default)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -85,7 +91,8 @@ def main():
 
     def step():
         return gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=mode, record_k=K, flags=flags,
-                               stream=stream.cuda_stream, block=args.block, waves_per_eu=args.waves_per_eu)
+                               stream=stream.cuda_stream, block=args.block, waves_per_eu=args.waves_per_eu,
+                               newton_batch=args.newton_batch)
 
     for _ in range(args.warmup):
         step()
@@ -127,7 +134,7 @@ def main():
                    "max_bounces": mb, "record_k": K, "newton_mode": args.mode, "seed_mode": "triangle",
                    "culling": not args.no_cull, "scene_seed": args.seed,
                    "schedule": "wavefront" if args.wavefront else "persistent", "block": args.block or 256,
-                   "waves_per_eu": args.waves_per_eu or 3},
+                   "waves_per_eu": args.waves_per_eu or 3, "newton_batch": args.newton_batch or 64},
         "roofline": {"bound": "mfma", "compute": "fp64 (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
@@ -140,6 +147,13 @@ def main():
         "scene_build_s": t_build,
     }
 
+    if not args.no_learned:
+        if mode != L.NEWTON_CONVERGED or args.wavefront:
+            res["learned"] = {"skipped": "the learned seed runs with converged Newton in the persistent schedule"}
+        else:
+            res["learned"] = learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value, res)
+            res["mean_newton_iters"]["learned"] = res["learned"]["mean_newton_iters"]
+
     traffic = committed_traffic(args, n, mb, K)
     if traffic is not None:
         res["roofline"]["traffic"] = traffic
@@ -147,6 +161,83 @@ def main():
     if args.cpu_seconds > 0:
         res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, mode, args.cpu_seconds, gs, K)
     print(json.dumps(res), flush=True)
+
+
+MLP_FLOPS_PER_WAVE_EVAL = 160 * 16 * 16 * 16 * 2   # 160 bf16 16x16x16 matrix products per 64-lane evaluation
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value_tri, res_tri):
+    """Trains the seed network on the GPU (untimed), then times the same trace with the learned seed and
+    checks its records against the triangle-seed run's."""
+    import numpy as np
+    import torch
+    from billardballraytracing3d_amd import tracer as T, _lib as L
+
+    kk = min(20, K)
+    ref = {k: outs[k].clone() for k in ("status", "err_kind", "bounces")}
+    if kk > 0:
+        ref["hit_patch"] = outs["hit_patch"][:, :kk].clone()
+        ref["hit_xyz"] = outs["hit_xyz"][:, :kk].clone()
+    net = T.Mlp(args.seed + 3)
+    t0 = time.perf_counter()
+    rep, curve = net.train(gs, n_samples=args.train_samples, seed=21 + args.seed, batch=args.train_batch,
+                           epochs=args.train_epochs, lr=args.train_lr, lr_final=args.train_lr / 100)
+    t_train = time.perf_counter() - t0
+    gs.set_mlp(net)
+
+    def step():
+        return gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=L.NEWTON_CONVERGED, record_k=K,
+                               flags=flags, stream=stream.cuda_stream, block=args.block,
+                               waves_per_eu=args.waves_per_eu, seed=L.SEED_LEARNED, newton_batch=args.newton_batch)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctr = np.zeros(L.NCOUNTERS, np.int64)
+    kms = np.zeros(3)
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        c, k = step()
+        ctr += c
+        kms += k
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    bounces = int(ctr[L.CTR["bounces"]])
+    value = bounces / (te - ts)
+    hits = int(ctr[L.CTR["seed_learned"]] + ctr[L.CTR["seed_fallback"]])
+    evals = int(ctr[L.CTR["mlp_evals"]])
+    # records against the triangle-seed run
+    same = (outs["status"] == ref["status"]) & (outs["err_kind"] == ref["err_kind"]) & (outs["bounces"] == ref["bounces"])
+    max_diff = 0.0
+    if kk > 0:
+        slot = torch.arange(kk, device=outs["bounces"].device)[None, :]
+        valid = slot < torch.clamp(ref["bounces"], max=kk)[:, None].to(slot.dtype)
+        same &= torch.all((outs["hit_patch"][:, :kk] == ref["hit_patch"]) | ~valid, dim=1)
+        diff = (outs["hit_xyz"][:, :kk] - ref["hit_xyz"]).abs().amax(dim=2)
+        max_diff = float(torch.where(valid, diff, torch.zeros_like(diff)).max().item())
+    mismatch = int((~same).sum().item())
+    kernel_ms = kms[0]
+    mlp_tflops = evals * MLP_FLOPS_PER_WAVE_EVAL / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    return {
+        "value": value, "unit": "bounces/s", "ms_per_step": (te - ts) * 1e3 / args.steps,
+        "vs_triangle": value / value_tri if value_tri else None,
+        "mean_newton_iters": int(ctr[L.CTR["newton_iters"]]) / hits if hits else None,
+        "stage1_hits_per_step": hits // args.steps,
+        "seed_learned_per_step": int(ctr[L.CTR["seed_learned"]]) // args.steps,
+        "seed_fallback_per_step": int(ctr[L.CTR["seed_fallback"]]) // args.steps,
+        "network_evals_per_step": evals // args.steps,
+        "lanes_per_eval": int(ctr[L.CTR["mlp_lanes"]]) / evals if evals else None,
+        "avg_launch_ms": kernel_ms / kms[1] if kms[1] else None,
+        "mlp_roofline": {"bound": "mfma", "compute": "bf16 dense", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": mlp_tflops / BF16_PEAK_TFLOPS},
+        "check_vs_triangle": {"rays": n, "bounces_compared": kk, "rays_with_mismatch": mismatch,
+                              "max_abs_hit_diff": max_diff},
+        "train": {"pairs": rep["n_pairs"], "steps": rep["steps"], "batch": args.train_batch,
+                  "epochs": args.train_epochs, "first_loss": rep["first_loss"], "final_loss": rep["final_loss"],
+                  "val_loss": rep["val_loss"], "test_loss": rep["test_loss"], "gen_ms": rep["gen_ms"],
+                  "train_ms": rep["train_ms"], "wall_s": t_train},
+    }
 
 
 def committed_traffic(args, n, mb, K):

@@ -21,18 +21,24 @@ ERR_NONE, ERR_SINGULAR, ERR_NOCONV, ERR_DOMAIN, ERR_BACKFACE, ERR_ZERO_DIR, ERR_
 FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS, FLAG_WAVEFRONT = 1, 2, 4, 8
 NCOUNTERS = 16
 CTR = dict(rays=0, bounces=1, bvh_node_tests=2, candidates=3, cull_box_tests=4, mt_tests=5, newton_calls=6,
-           newton_iters=7, newton_flops=8, polish_fail=9, launches=10)
+           newton_iters=7, newton_flops=8, polish_fail=9, launches=10, seed_learned=11, seed_fallback=12,
+           mlp_evals=13, mlp_lanes=14)
+MLP_NIN = 51
+MLP_NPARAMS = 13520
 
 # Every symbol include/bbrt.h declares.
 EXPORTS = ("bbrt_scene_create", "bbrt_scene_destroy", "bbrt_scene_info", "bbrt_scene_patch_info", "bbrt_scene_bvh",
            "bbrt_scene_coarse", "bbrt_trace", "bbrt_trace_device", "bbrt_prepare_patches", "bbrt_build_bvh",
-           "bbrt_last_error")
+           "bbrt_last_error", "bbrt_mlp_create", "bbrt_mlp_destroy", "bbrt_mlp_get_params", "bbrt_mlp_set_params",
+           "bbrt_mlp_forward", "bbrt_mlp_loss_grad", "bbrt_mlp_train", "bbrt_mlp_sample_pairs", "bbrt_mlp_features",
+           "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab")
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _i64 = C.POINTER(C.c_int64)
 _i8 = C.POINTER(C.c_int8)
 _u8 = C.POINTER(C.c_uint8)
+_f32 = C.POINTER(C.c_float)
 
 
 class PatchSet(C.Structure):
@@ -42,13 +48,25 @@ class PatchSet(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("newton_mode", C.c_int32), ("seed_mode", C.c_int32), ("max_bounces", C.c_int32),
-                ("record_k", C.c_int32), ("flags", C.c_uint32), ("block", C.c_int32), ("waves_per_eu", C.c_int32)]
+                ("record_k", C.c_int32), ("flags", C.c_uint32), ("block", C.c_int32), ("waves_per_eu", C.c_int32),
+                ("newton_batch", C.c_int32)]
 
 
 class Outputs(C.Structure):
     _fields_ = [("status", C.c_void_p), ("err_kind", C.c_void_p), ("bounces", C.c_void_p), ("err_patch", C.c_void_p),
                 ("hit_patch", C.c_void_p), ("hit_xyz", C.c_void_p), ("newton_iters", C.c_void_p),
                 ("final_od", C.c_void_p), ("hit_count", C.c_void_p), ("ray_work", C.c_void_p)]
+
+
+class TrainConfig(C.Structure):
+    _fields_ = [("n_samples", C.c_int64), ("seed", C.c_uint64), ("batch", C.c_int32), ("epochs", C.c_int32),
+                ("lr", C.c_float), ("lr_final", C.c_float), ("val_frac", C.c_float), ("test_frac", C.c_float)]
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("n_pairs", C.c_int64), ("n_train", C.c_int64), ("n_val", C.c_int64), ("n_test", C.c_int64),
+                ("steps", C.c_int64), ("first_loss", C.c_float), ("final_loss", C.c_float), ("val_loss", C.c_float),
+                ("test_loss", C.c_float), ("gen_ms", C.c_double), ("train_ms", C.c_double)]
 
 
 class BBRTError(RuntimeError):
@@ -79,6 +97,19 @@ def lib():
         L.bbrt_prepare_patches.argtypes = [C.POINTER(PatchSet), _d, _d, _d, _d, _d]
         L.bbrt_build_bvh.argtypes = [_d, _d, C.c_int64, C.c_int32, _d, _i32, _i32, _i32, _u8, _i32]
         L.bbrt_build_bvh.restype = C.c_int64
+        _f = C.POINTER(C.c_float)
+        L.bbrt_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+        L.bbrt_mlp_destroy.argtypes = [C.c_void_p]
+        L.bbrt_mlp_destroy.restype = None
+        L.bbrt_mlp_get_params.argtypes = [C.c_void_p, _f]
+        L.bbrt_mlp_set_params.argtypes = [C.c_void_p, _f]
+        L.bbrt_mlp_forward.argtypes = [C.c_void_p, _f, C.c_int64, _f]
+        L.bbrt_mlp_loss_grad.argtypes = [C.c_void_p, _f, _f, C.c_int64, _f, _f]
+        L.bbrt_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(TrainConfig), _f, C.POINTER(TrainReport)]
+        L.bbrt_mlp_sample_pairs.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, _i32, _d, _d, _f, _f, _i8]
+        L.bbrt_mlp_features.argtypes = [C.c_void_p, _i32, _d, _d, C.c_int64, _f, _d, _d]
+        L.bbrt_scene_set_mlp.argtypes = [C.c_void_p, C.c_void_p]
+        L.bbrt_mlp_seed_ab.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, _i8, _u8, _u8, _f, _d]
         _LIB = L
     return _LIB
 

@@ -33,11 +33,14 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     o_host = os.path.join(BUILD, "scene_build.o")
-    o_dev = os.path.join(BUILD, "bbrt_hip.o")
     _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", o_host], verbose)
-    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "bbrt_hip.hip"), "-o", o_dev], verbose)
+    objs = []
+    for name in ("bbrt_hip", "bbrt_mlp"):
+        o = os.path.join(BUILD, name + ".o")
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, name + ".hip"), "-o", o], verbose)
+        objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp, o_dev, o_host], verbose)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + [o_host], verbose)
     os.replace(tmp, LIB)
     return LIB
 

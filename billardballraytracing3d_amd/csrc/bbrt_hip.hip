@@ -33,6 +33,7 @@
 #include <thread>
 #include <vector>
 
+#include "mlp_device.h"
 #include "runtime.h"
 
 thread_local std::string bbrt::g_last_error;
@@ -101,12 +102,29 @@ __global__ void k_init(const double* __restrict__ origins, const double* __restr
 // bounce at once -- a lane never waits for another lane's whole bounce.  Every
 // ray ends within max_bounces bounces and the list is finite, so every wave
 // leaves the loop.
-enum : int { kPhTrav = 1, kPhLeaf = 2, kPhDone = 3 };
+enum : int { kPhTrav = 1, kPhLeaf = 2, kPhDone = 3, kPhSeed = 4 };
 
-template <int W, bool WORK>
+// A lane whose stage 1 hits parks (kPhSeed) with its triangle seed; once
+// newton_batch lanes of the wave are parked, or no other lane can make
+// progress, they run Newton together, so Newton's FP64 work is spread over
+// many lanes instead of one or two (the results do not depend on the
+// schedule).  LEARNED: at that point the wave first evaluates the network for
+// the parked lanes; Newton starts from the network's seed and its result is
+// kept only under the acceptance rule (converged, polished, within
+// mlp::kSeedWindow of the triangle seed), else Newton runs again from the
+// triangle seed and its outcome stands.
+template <int W, bool WORK, bool LEARNED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_t* __restrict__ n_live_p,
-          int32_t* head, Outs O, int max_bounces, int record_k, int mode, uint32_t flags, unsigned long long* ctr) {
+          int32_t* head, Outs O, int max_bounces, int record_k, int mode, uint32_t flags, unsigned long long* ctr,
+          MlpDev MD, int newton_batch) {
+    const MlpLds* lds = nullptr;
+    if constexpr (LEARNED) {
+        __shared__ MlpLds mlp_lds;
+        mlp_stage_lds(MD, mlp_lds, threadIdx.x, blockDim.x);
+        __syncthreads();
+        lds = &mlp_lds;
+    }
     const int n_live = *n_live_p;
     const int lane = __lane_id();
     Counters c = {};
@@ -126,6 +144,8 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
     int err = 0;
     int leaf_start = 0, leaf_count = 0, q = 0;
     uint32_t enter = 0;
+    int32_t obj = 0;                          // the leaf object being intersected
+    Seed sd{ 0.0, 0.0, 0.0, false };          // its stage-1 seed (learned mode, while parked)
 
     // bounce set-up: RayTracingDriver.hpp:187 and the root of BVH.hpp:89
     auto start_bounce = [&](bool root_live) {
@@ -209,15 +229,48 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         }
 
         // one leaf object: SurfacePatch::intersect and the best-hit update (BVH.hpp:101-109)
+        bool have_h = false;
+        PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
         if (r >= 0 && phase == kPhLeaf) {
-            const int32_t obj = S.leaf[leaf_start + q];
+            obj = S.leaf[leaf_start + q];
+            have_h = true;
             if (enter & (1u << q)) {
-                const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
-                if (h.status < 0) { err = -h.status; err_obj = obj; phase = kPhDone; }
-                else if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
+                sd = stage1_seed(S, obj, o, d, r32, cull, c);
+                if (sd.have) { phase = kPhSeed; have_h = false; }   // parked until the wave runs Newton
             } else {
                 c.candidates++;
             }
+        }
+        // Newton for the parked lanes, all together: once newton_batch lanes wait, or
+        // when no other lane can make progress
+        const unsigned long long pend = __ballot(r >= 0 && phase == kPhSeed);
+        const unsigned long long busy = __ballot((r >= 0 && phase != kPhSeed) || r == -1);
+        if (pend && ((int)__popcll(pend) >= newton_batch || busy == 0)) {
+            const bool need = r >= 0 && phase == kPhSeed;
+            if constexpr (LEARNED) {
+                double ul, vl, tl;
+                mlp_seed_wave(MD, *lds, need, obj, o, d, ul, vl, tl);
+                if (lane == 0) c.mlp_evals++;
+                if (need) {
+                    c.mlp_lanes++;
+                    h = newton_any(S, obj, o, d, ul, vl, tl, mode, c);
+                    const bool ok = h.status == 1 && h.polish_ok && fabs(h.u - sd.u) <= mlp::kSeedWindow &&
+                                    fabs(h.v - sd.v) <= mlp::kSeedWindow;
+                    if (ok) {
+                        c.seed_learned++;
+                    } else {
+                        c.seed_fallback++;
+                        h = newton_any(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
+                    }
+                }
+            } else {
+                if (need) h = newton_any(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
+            }
+            if (need) { phase = kPhLeaf; have_h = true; }
+        }
+        if (have_h) {
+            if (h.status < 0) { err = -h.status; err_obj = obj; phase = kPhDone; }
+            else if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
             if (phase == kPhLeaf && ++q == leaf_count) {
                 if (sp == 0) phase = kPhDone;
                 else { cur = stack[--sp]; phase = kPhTrav; }
@@ -284,8 +337,17 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     if (p->max_bounces < 0 || p->record_k < 0) throw std::runtime_error("bbrt_trace: negative max_bounces or record_k");
     if (p->newton_mode != BBRT_NEWTON_REF && p->newton_mode != BBRT_NEWTON_CONVERGED)
         throw std::runtime_error("bbrt_trace: unknown newton_mode");
-    if (p->seed_mode != BBRT_SEED_TRIANGLE)
-        throw std::runtime_error("bbrt_trace: only the triangle (stage-1) seed is available in this library");
+    if (p->seed_mode != BBRT_SEED_TRIANGLE && p->seed_mode != BBRT_SEED_LEARNED)
+        throw std::runtime_error("bbrt_trace: unknown seed_mode");
+    const bool learned = p->seed_mode == BBRT_SEED_LEARNED;
+    if (learned) {
+        if (!s->mlp_ready) throw std::runtime_error("bbrt_trace: the learned seed needs a network (bbrt_scene_set_mlp)");
+        if (p->newton_mode != BBRT_NEWTON_CONVERGED)
+            throw std::runtime_error("bbrt_trace: the learned seed needs BBRT_NEWTON_CONVERGED");
+        if (p->flags & BBRT_FLAG_WAVEFRONT)
+            throw std::runtime_error("bbrt_trace: the learned seed runs in the persistent schedule only");
+        if (out && out->ray_work) throw std::runtime_error("bbrt_trace: ray_work is not recorded with the learned seed");
+    }
     if (n < 0 || n > (int64_t)INT32_MAX - 1024) throw std::runtime_error("bbrt_trace: ray count out of range");
     const int block = p->block > 0 ? p->block : 256;
     if (block % 64 != 0 || block > 256) throw std::runtime_error("bbrt_trace: block must be a multiple of 64 <= 256");
@@ -340,21 +402,32 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     if (!wavefront) {
         if (p->max_bounces > 0) {
             const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : kDefaultWavesPerEu;
-            decltype(&k_persist<1, false>) kern = nullptr;
+            decltype(&k_persist<2, false, false>) kern = nullptr;
             const bool work = O.ray_work != nullptr;
-            switch (wpe) {
-                case 1: case 2: kern = work ? &k_persist<1, true> : &k_persist<1, false>; break;
-                case 3: kern = work ? &k_persist<3, true> : &k_persist<3, false>; break;
-                case 4: kern = work ? &k_persist<4, true> : &k_persist<4, false>; break;
-                default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
+            if (learned) {
+                switch (wpe) {
+                    case 2: kern = &k_persist<2, false, true>; break;
+                    case 3: kern = &k_persist<3, false, true>; break;
+                    case 4: kern = &k_persist<4, false, true>; break;
+                    default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
+                }
+            } else {
+                switch (wpe) {
+                    case 2: kern = work ? &k_persist<2, true, false> : &k_persist<2, false, false>; break;
+                    case 3: kern = work ? &k_persist<3, true, false> : &k_persist<3, false, false>; break;
+                    case 4: kern = work ? &k_persist<4, true, false> : &k_persist<4, false, false>; break;
+                    default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
+                }
             }
+            const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
             int per_cu = 0;
             check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0), "occupancy");
             const int64_t resident = (int64_t)std::max(1, per_cu) * s->n_cu;
             const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, resident));
             if (timing) check(hipEventRecord(s->kev[0], st), "hipEventRecord");
             hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s->dev, R, live[0], cnt, cnt + 2, O,
-                               p->max_bounces, p->record_k, p->newton_mode, p->flags, ctr);
+                               p->max_bounces, p->record_k, p->newton_mode, p->flags, ctr, MD,
+                               p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch);
             if (timing) check(hipEventRecord(s->kev[1], st), "hipEventRecord");
             check(hipGetLastError(), "k_persist launch");
             launches = 1;

@@ -15,6 +15,7 @@ namespace {
 
 constexpr int kStackCap = 48;
 constexpr int kDefaultWavesPerEu = 3;         // persistent-kernel occupancy target (best of 2/3/4 on MI355X)
+constexpr int kDefaultNewtonBatch = 64;       // parked lanes (stage-1 hits) that trigger a batched Newton (64: when no lane can go on)
 constexpr double kMtEps = 1e-9;               // MollerTrumbore.hpp:33
 constexpr double kOnNodeTol = 1.0e-12;        // PatchInterpolation.hpp:21
 constexpr int kNewtonMaxIters = 20;           // SurfacePatch.hpp:121
@@ -82,6 +83,7 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 struct Counters {
     uint32_t bvh_tests, candidates, cull_tests, mt_tests, newton_calls, newton_iters, newton_flops, polish_fail,
         bounces;
+    uint32_t seed_learned, seed_fallback, mlp_evals, mlp_lanes;   // learned-seed mode only
 };
 
 // MollerTrumbore.hpp:32-64 (t only; the point and normal are unused by SurfacePatch).
@@ -241,6 +243,8 @@ struct PHit {
     double t;
     V3 p, n;
     int iters;
+    double u = 0.0, v = 0.0;   // the converged parameters of a hit
+    bool polish_ok = true;     // converged mode: the polish met its tolerance
 };
 
 // SurfacePatch.hpp:518-538
@@ -398,42 +402,63 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
             if (sn < kPolishRelTol * smax(1.0, fabs(t))) { ok = true; break; }
         }
         if (!ok) c.polish_fail++;
+        r.polish_ok = ok;
     }
     c.newton_iters += iters; c.newton_flops += iters * I_it + F_fin;
     basis<N>(un, uw, im, u, lu);
     basis<N>(vn, vw, jm, v, lv);
-    r.status = 1; r.t = t; r.iters = iters;
+    r.status = 1; r.t = t; r.iters = iters; r.u = u; r.v = v;
     r.p = { field<N>(lu, lv, F, im, jm), field<N>(lu, lv, F + nn, im, jm), field<N>(lu, lv, F + 2 * nn, im, jm) };
     r.n = unit3({ field<N>(lu, lv, F + 9 * nn, im, jm), field<N>(lu, lv, F + 10 * nn, im, jm),
                   field<N>(lu, lv, F + 11 * nn, im, jm) });
     return r;
 }
 
-// SurfacePatch::intersect (SurfacePatch.hpp:253-303)
-__device__ PHit patch_intersect(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32, bool cull,
-                                int mode, Counters& c) {
-    const PatchMeta M = S.meta[k];
+// Stage 1 of SurfacePatch::intersect (SurfacePatch.hpp:272-298) and the
+// triangle-centroid seed (triangle_uv_centroid, :499-510; coarse_.u[i] =
+// -1 + 2i/99, :421).  have = false when no sub-triangle is hit (a miss).
+struct Seed {
+    double u, v, t;
+    bool have;
+};
+
+__device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32,
+                                            bool cull, Counters& c) {
     const double* G = S.cxyz + (size_t)k * bbrt::kCoarseN * bbrt::kCoarseN * 3;
     const double* C = S.cell + (size_t)k * bbrt::kCells * bbrt::kCells * 4;
     Stage1 best{ 0.0, 0, false };
     c.candidates++;
-    if (cull && M.cull) stage1_culled(S, k, G, C, o, d, r32, best, c);
+    if (cull && S.meta[k].cull) stage1_culled(S, k, G, C, o, d, r32, best, c);
     else stage1_brute(G, C, o, d, best, c);
-    if (!best.have) return PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+    Seed sd{ 0.0, 0.0, best.t, best.have };
+    if (!best.have) return sd;
     const int cellk = best.key / 6;
     const int i = cellk / bbrt::kCells, j = cellk % bbrt::kCells;
     const bool second = (best.key & 1) != 0;
-    // coarse_.u[i] = -1 + 2 i / 99 (SurfacePatch.hpp:421), then triangle_uv_centroid (:499-510)
     const double ui = -1.0 + 2.0 * static_cast<double>(i) / (bbrt::kCoarseN - 1);
     const double uip1 = -1.0 + 2.0 * static_cast<double>(i + 1) / (bbrt::kCoarseN - 1);
     const double vj = -1.0 + 2.0 * static_cast<double>(j) / (bbrt::kCoarseN - 1);
     const double vjp1 = -1.0 + 2.0 * static_cast<double>(j + 1) / (bbrt::kCoarseN - 1);
-    double u0, v0;
-    if (!second) { u0 = (ui + uip1 + uip1) / 3.0; v0 = (vj + vj + vjp1) / 3.0; }
-    else { u0 = (ui + ui + uip1) / 3.0; v0 = (vj + vjp1 + vjp1) / 3.0; }
-    if (M.imax == 3 && M.jmax == 3) return newton<3>(S, M, o, d, u0, v0, best.t, mode, c);
-    if (M.imax == 5 && M.jmax == 5) return newton<5>(S, M, o, d, u0, v0, best.t, mode, c);
-    return newton<0>(S, M, o, d, u0, v0, best.t, mode, c);
+    if (!second) { sd.u = (ui + uip1 + uip1) / 3.0; sd.v = (vj + vj + vjp1) / 3.0; }
+    else { sd.u = (ui + ui + uip1) / 3.0; sd.v = (vj + vjp1 + vjp1) / 3.0; }
+    return sd;
+}
+
+// newton_refine from (u0, v0, t0) on patch k, specialised on the node count.
+__device__ __forceinline__ PHit newton_any(const DevScene& S, int64_t k, const V3& o, const V3& d, double u0, double v0,
+                                           double t0, int mode, Counters& c) {
+    const PatchMeta M = S.meta[k];
+    if (M.imax == 3 && M.jmax == 3) return newton<3>(S, M, o, d, u0, v0, t0, mode, c);
+    if (M.imax == 5 && M.jmax == 5) return newton<5>(S, M, o, d, u0, v0, t0, mode, c);
+    return newton<0>(S, M, o, d, u0, v0, t0, mode, c);
+}
+
+// SurfacePatch::intersect (SurfacePatch.hpp:253-303): stage 1, then Newton from its seed.
+__device__ PHit patch_intersect(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32, bool cull,
+                                int mode, Counters& c) {
+    const Seed sd = stage1_seed(S, k, o, d, r32, cull, c);
+    if (!sd.have) return PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+    return newton_any(S, k, o, d, sd.u, sd.v, sd.t, mode, c);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -443,12 +468,14 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ __forceinline__ void flush_counters(const Counters& c, unsigned long long* ctr) {
-    const unsigned long long v[9] = { c.bvh_tests, c.candidates, c.cull_tests, c.mt_tests, c.newton_calls,
-                                      c.newton_iters, c.newton_flops, c.polish_fail, c.bounces };  // widened
-    const int slot[9] = { bbrt::kCtrBvh, bbrt::kCtrCand, bbrt::kCtrCull, bbrt::kCtrMt, bbrt::kCtrNewtonCalls,
-                          bbrt::kCtrNewtonIters, bbrt::kCtrNewtonFlops, bbrt::kCtrPolishFail, bbrt::kCtrBounces };
+    const unsigned long long v[13] = { c.bvh_tests, c.candidates, c.cull_tests, c.mt_tests, c.newton_calls,
+                                       c.newton_iters, c.newton_flops, c.polish_fail, c.bounces, c.seed_learned,
+                                       c.seed_fallback, c.mlp_evals, c.mlp_lanes };  // widened
+    const int slot[13] = { bbrt::kCtrBvh, bbrt::kCtrCand, bbrt::kCtrCull, bbrt::kCtrMt, bbrt::kCtrNewtonCalls,
+                           bbrt::kCtrNewtonIters, bbrt::kCtrNewtonFlops, bbrt::kCtrPolishFail, bbrt::kCtrBounces,
+                           BBRT_CTR_SEED_LEARNED, BBRT_CTR_SEED_FALLBACK, BBRT_CTR_MLP_EVALS, BBRT_CTR_MLP_LANES };
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
+    for (int q = 0; q < 13; ++q) {
         const unsigned long long s = wave_sum(v[q]);
         if (__lane_id() == 0 && s) atomicAdd(ctr + slot[q], s);
     }

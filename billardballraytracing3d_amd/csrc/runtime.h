@@ -59,6 +59,14 @@ struct bbrt_scene_s {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     int32_t* h_count = nullptr;   // pinned
+    // learned seed: per-patch frames (fp64 [n][16]) and features (fp32 [n][48]),
+    // and the attached network's snapshot (bbrt_scene_set_mlp)
+    double* mlp_frame = nullptr;
+    float* mlp_pfeat = nullptr;
+    float* mlp_h1pre = nullptr;
+    void* mlp_wfrag = nullptr;
+    float* mlp_params = nullptr;
+    bool mlp_ready = false;
     hipEvent_t ev[2] = { nullptr, nullptr };
     std::vector<hipEvent_t> kev;
 

@@ -361,6 +361,12 @@ def sinai_scene(seed: int = 0, freq: int = 16, order: int = 4) -> Scene:
     return build_scene(sinai_obstacles(seed, freq, order))
 
 
+def curved_ellipsoid_scene(seed: int = 0, freq: int = 16, order: int = 4) -> Scene:
+    """One high-curvature ellipsoid, semi-axes (5, 1, 0.5), 2,048 order-4 patches (octahedral frequency 16)."""
+    ob = Obstacle("ellipsoid", np.zeros(3), np.array([5.0, 1.0, 0.5]), "oct", freq, order, rot_seed=(seed << 8) + 40)
+    return build_scene([ob])
+
+
 def plane_grid(corner0, corner1, nx: int, ny: int) -> np.ndarray:
     """Inclusive nx x ny grid between two corners on an axis-aligned plane.
 

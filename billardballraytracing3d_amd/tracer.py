@@ -108,7 +108,8 @@ class GpuScene:
 
     def trace(self, origins, dirs, max_bounces: int, mode: int = L.NEWTON_CONVERGED,
               record_k: Optional[int] = None, flags: int = 0, hit_count: bool = False, block: int = 0,
-              ray_work: bool = False, waves_per_eu: int = 0) -> TraceResult:
+              ray_work: bool = False, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE,
+              newton_batch: int = 0) -> TraceResult:
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
         d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
         if o.shape != d.shape:
@@ -120,7 +121,7 @@ class GpuScene:
                           np.zeros((n, 6)), np.zeros(L.NCOUNTERS, np.int64),
                           np.zeros(self.n_patches, np.int64) if hit_count else None,
                           np.zeros((n, 4), np.uint32) if ray_work else None)
-        p = L.Params(mode, L.SEED_TRIANGLE, max_bounces, K, flags, block, waves_per_eu)
+        p = L.Params(mode, seed, max_bounces, K, flags, block, waves_per_eu, newton_batch)
         out = L.Outputs(L.vptr(res.status), L.vptr(res.err_kind), L.vptr(res.bounces), L.vptr(res.err_patch),
                         L.vptr(res.hit_patch), L.vptr(res.hit_xyz), L.vptr(res.iters), L.vptr(res.final_od),
                         L.vptr(res.hit_count), L.vptr(res.ray_work))
@@ -131,9 +132,9 @@ class GpuScene:
 
     def trace_device(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
                      mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0, stream: int = 0,
-                     block: int = 0, waves_per_eu: int = 0):
+                     block: int = 0, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE, newton_batch: int = 0):
         """Trace with device-resident inputs/outputs (raw device addresses). Returns (counters, kernel_ms[3])."""
-        p = L.Params(mode, L.SEED_TRIANGLE, max_bounces, record_k, flags, block, waves_per_eu)
+        p = L.Params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newton_batch)
         out = L.Outputs(*[C.c_void_p(out_ptrs.get(k) or 0) if out_ptrs.get(k) else None
                           for k in ("status", "err_kind", "bounces", "err_patch", "hit_patch", "hit_xyz",
                                     "newton_iters", "final_od", "hit_count", "ray_work")])
@@ -144,3 +145,101 @@ class GpuScene:
                                        C.c_void_p(stream) if stream else None, L.ptr(kms, L._d))
         L.check(rc, "bbrt_trace_device")
         return counters, kms
+
+    # ------------------------------------------------------------------
+    # learned seed
+    # ------------------------------------------------------------------
+    def set_mlp(self, mlp: "Mlp"):
+        """Attach a snapshot of the network (bbrt_scene_set_mlp); SEED_LEARNED traces use it."""
+        L.check(L.lib().bbrt_scene_set_mlp(self.h, mlp.h), "bbrt_scene_set_mlp")
+
+    def sample_pairs(self, seed: int, first: int, n: int):
+        """(patch, origin, dir, x[51], target[3], valid) of training pairs first .. first+n-1."""
+        patch = np.zeros(n, np.int32); o = np.zeros((n, 3)); d = np.zeros((n, 3))
+        x = np.zeros((n, L.MLP_NIN), np.float32); t = np.zeros((n, 3), np.float32); v = np.zeros(n, np.int8)
+        L.check(L.lib().bbrt_mlp_sample_pairs(self.h, C.c_uint64(seed), C.c_int64(first), C.c_int64(n),
+                                              L.ptr(patch, L._i32), L.ptr(o, L._d), L.ptr(d, L._d),
+                                              L.ptr(x, L._f32), L.ptr(t, L._f32), L.ptr(v, L._i8)),
+                "bbrt_mlp_sample_pairs")
+        return patch, o, d, x, t, v.astype(bool)
+
+    def mlp_features(self, patch, origins, dirs, frame: bool = False):
+        patch = np.ascontiguousarray(patch, dtype=np.int32)
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = patch.size
+        x = np.zeros((n, L.MLP_NIN), np.float32); tc = np.zeros((n, 3))
+        fr = np.zeros((self.n_patches, 24)) if frame else None
+        L.check(L.lib().bbrt_mlp_features(self.h, L.ptr(patch, L._i32), L.ptr(o, L._d), L.ptr(d, L._d),
+                                          C.c_int64(n), L.ptr(x, L._f32), L.ptr(tc, L._d),
+                                          L.ptr(fr, L._d) if frame else None), "bbrt_mlp_features")
+        return (x, tc, fr) if frame else (x, tc)
+
+    def seed_ab(self, seed: int, first: int, n: int):
+        """Triangle seed vs learned seed on sampled pairs (bbrt_mlp_seed_ab)."""
+        st = np.zeros(2 * n, np.int8); it = np.zeros(2 * n, np.uint8); ac = np.zeros(n, np.uint8)
+        se = np.zeros(2 * n, np.float32); pd = np.zeros(n)
+        L.check(L.lib().bbrt_mlp_seed_ab(self.h, C.c_uint64(seed), C.c_int64(first), C.c_int64(n),
+                                         L.ptr(st, L._i8), L.ptr(it, L._u8), L.ptr(ac, L._u8), L.ptr(se, L._f32),
+                                         L.ptr(pd, L._d)), "bbrt_mlp_seed_ab")
+        return dict(status_tri=st[:n], status_learned=st[n:], iters_tri=it[:n], iters_learned=it[n:],
+                    accepted=ac.astype(bool), seed_err_tri=se[:n], seed_err_learned=se[n:], point_diff=pd)
+
+
+class Mlp:
+    """The learned-seed network (bbrt_mlp): GPU forward, gradients and training."""
+
+    def __init__(self, seed: int = 0):
+        h = C.c_void_p()
+        L.check(L.lib().bbrt_mlp_create(C.c_uint64(seed), C.byref(h)), "bbrt_mlp_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().bbrt_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(L.MLP_NPARAMS, np.float32)
+        L.check(L.lib().bbrt_mlp_get_params(self.h, L.ptr(p, L._f32)), "bbrt_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != L.MLP_NPARAMS:
+            raise ValueError("wrong parameter count")
+        L.check(L.lib().bbrt_mlp_set_params(self.h, L.ptr(p, L._f32)), "bbrt_mlp_set_params")
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, L.MLP_NIN)
+        y = np.zeros((x.shape[0], 3), np.float32)
+        L.check(L.lib().bbrt_mlp_forward(self.h, L.ptr(x, L._f32), C.c_int64(x.shape[0]), L.ptr(y, L._f32)),
+                "bbrt_mlp_forward")
+        return y
+
+    def loss_grad(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, L.MLP_NIN)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, 3)
+        g = np.zeros(L.MLP_NPARAMS, np.float32); loss = C.c_float()
+        L.check(L.lib().bbrt_mlp_loss_grad(self.h, L.ptr(x, L._f32), L.ptr(t, L._f32), C.c_int64(x.shape[0]),
+                                           C.byref(loss), L.ptr(g, L._f32)), "bbrt_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, scene: GpuScene, n_samples: int, seed: int = 1, batch: int = 16384, epochs: int = 4,
+              lr: float = 2e-3, lr_final: float = 2e-5, val_frac: float = 0.1, test_frac: float = 0.1):
+        cfg = L.TrainConfig(n_samples, seed, batch, epochs, lr, lr_final, val_frac, test_frac)
+        rep = L.TrainReport()
+        n_train_max = int(n_samples * (1.0 - val_frac - test_frac)) + 1
+        curve = np.zeros(max(1, (n_train_max // batch) * epochs + 1), np.float32)
+        L.check(L.lib().bbrt_mlp_train(self.h, scene.h, C.byref(cfg), L.ptr(curve, L._f32), C.byref(rep)),
+                "bbrt_mlp_train")
+        report = {k: getattr(rep, k) for k, _ in L.TrainReport._fields_}
+        return report, curve[:rep.steps]

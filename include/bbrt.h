@@ -68,13 +68,16 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
 
 typedef struct {
     int32_t newton_mode;   /* BBRT_NEWTON_* */
-    int32_t seed_mode;     /* BBRT_SEED_* (only TRIANGLE is implemented) */
+    int32_t seed_mode;     /* BBRT_SEED_*; LEARNED needs BBRT_NEWTON_CONVERGED and bbrt_scene_set_mlp */
     int32_t max_bounces;   /* run(max_bounces), RayTracingDriver.hpp:143 */
     int32_t record_k;      /* bounces recorded per ray in hit_patch / hit_xyz / newton_iters */
     uint32_t flags;        /* BBRT_FLAG_* */
     int32_t block;         /* threads per block of the bounce kernel; 0 = library default */
     int32_t waves_per_eu;  /* occupancy target of the persistent kernel: 0 = library default, or 2, 3, 4
                               (the compiler keeps registers within that occupancy, spilling if needed) */
+    int32_t newton_batch;  /* persistent kernel: lanes of a wave whose stage 1 hit wait until this many can run
+                              Newton (and the learned seed's network) together (1..64; 0 = library default, 64);
+                              results do not depend on it */
 } bbrt_params;
 
 /* Outputs; any pointer may be NULL.  Host memory for bbrt_trace, device
@@ -110,7 +113,11 @@ typedef struct {
 /* Work counters returned by the trace calls (int64[BBRT_NCOUNTERS]). */
 enum { BBRT_CTR_RAYS = 0, BBRT_CTR_BOUNCES = 1, BBRT_CTR_BVH_NODE_TESTS = 2, BBRT_CTR_CANDIDATES = 3,
        BBRT_CTR_CULL_BOX_TESTS = 4, BBRT_CTR_MT_TESTS = 5, BBRT_CTR_NEWTON_CALLS = 6, BBRT_CTR_NEWTON_ITERS = 7,
-       BBRT_CTR_NEWTON_FLOPS = 8, BBRT_CTR_POLISH_FAIL = 9, BBRT_CTR_LAUNCHES = 10, BBRT_NCOUNTERS = 16 };
+       BBRT_CTR_NEWTON_FLOPS = 8, BBRT_CTR_POLISH_FAIL = 9, BBRT_CTR_LAUNCHES = 10,
+       /* learned-seed mode: stage-1 hits whose learned-seed Newton was accepted, those that fell back to the
+          triangle seed, wave-level network evaluations, and lanes seeded by them */
+       BBRT_CTR_SEED_LEARNED = 11, BBRT_CTR_SEED_FALLBACK = 12, BBRT_CTR_MLP_EVALS = 13, BBRT_CTR_MLP_LANES = 14,
+       BBRT_NCOUNTERS = 16 };
 
 /* Builds a scene: per patch the barycentric weights, centroid, node box,
  * 100x100 coarse grid, per-cell sagitta and normal and the inflated box
@@ -163,6 +170,85 @@ int bbrt_prepare_patches(const bbrt_patch_set* patches, double* box, double* cen
 int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int32_t leaf_threshold,
                        double* node_box, int32_t* right, int32_t* start, int32_t* count, uint8_t* axis,
                        int32_t* leaf_objects);
+
+/* ------------------------------------------------------------------------
+ * Learned Newton seed.  No counterpart in the reference, whose Newton seed is
+ * always the centroid of the closest stage-1 triangle
+ * (SurfacePatch.hpp:272-303, :499-510).  A 51 -> 64 -> 64 -> 64 -> 3 GELU
+ * network predicts the converged (u, v, t) of a (ray, patch) pair from the ray
+ * in the patch's local frame (6 values) and 15 points of the patch (45
+ * values).  It is trained on the GPU (bf16 matrix cores, fp32 accumulation,
+ * Adam on fp32 master weights) on pairs that the method itself solves.
+ * With seed_mode BBRT_SEED_LEARNED (converged Newton mode only) stage 1 still
+ * decides hit or miss; for a hit, Newton starts from the network's seed, and
+ * its result is kept only if it converges (classification and polish) to a
+ * root within two coarse cells of the triangle seed.  Otherwise Newton runs
+ * again from the triangle seed, whose outcome stands -- so every result is
+ * one the triangle-seed path can produce, whatever the network says.
+ * Parameters (fp32, BBRT_MLP_NPARAMS): W1[64][64] (inputs 51..63 zero), b1[64],
+ * W2[64][64], b2[64], W3[64][64], b3[64], W4[16][64] (rows 3..15 zero),
+ * b4[16] (3..15 zero).  Feature order: ray (6) then patch (45).
+ * ------------------------------------------------------------------------ */
+#define BBRT_MLP_NIN 51
+#define BBRT_MLP_NPARAMS 13520
+typedef struct bbrt_mlp_s* bbrt_mlp;
+
+/* Glorot-uniform weights from SplitMix64(seed), zero biases. */
+int bbrt_mlp_create(uint64_t seed, bbrt_mlp* out);
+void bbrt_mlp_destroy(bbrt_mlp mlp);
+int bbrt_mlp_get_params(bbrt_mlp mlp, float* params);
+int bbrt_mlp_set_params(bbrt_mlp mlp, const float* params);   /* pads must be zero */
+/* Forward pass on the GPU: x [n][51] -> y [n][3] (host arrays). */
+int bbrt_mlp_forward(bbrt_mlp mlp, const float* x, int64_t n, float* y);
+/* Mean squared error over the batch and its gradient [BBRT_MLP_NPARAMS], by
+ * the training kernels (host arrays, n <= 1<<20). */
+int bbrt_mlp_loss_grad(bbrt_mlp mlp, const float* x, const float* target, int64_t n, float* loss, float* grad);
+
+typedef struct {
+    int64_t n_samples;   /* (ray, patch) pairs drawn; pairs stage 1 misses are dropped */
+    uint64_t seed;
+    int32_t batch;       /* samples per Adam step, a multiple of 64 */
+    int32_t epochs;
+    float lr, lr_final;  /* cosine decay from lr to lr_final */
+    float val_frac, test_frac;
+} bbrt_train_config;
+typedef struct {
+    int64_t n_pairs;     /* solved pairs kept (train + validation + test) */
+    int64_t n_train, n_val, n_test;
+    int64_t steps;
+    float first_loss, final_loss;   /* training loss of the first and last step */
+    float val_loss, test_loss;
+    double gen_ms, train_ms;
+} bbrt_train_report;
+/* Draws cfg->n_samples pairs from `scene` (patch uniform, (u, v) uniform in
+ * [-1, 1]^2, direction uniform over the front hemisphere, origin 0.25..4
+ * patch scales back along it), solves each with stage 1 + converged Newton,
+ * and trains.  loss_curve (may be NULL) receives one value per step. */
+int bbrt_mlp_train(bbrt_mlp mlp, bbrt_scene scene, const bbrt_train_config* cfg, float* loss_curve,
+                   bbrt_train_report* report);
+/* Pairs as the trainer draws them (sample indices first .. first+n-1): patch,
+ * origin, direction, features x [n][51] (fp32 values of the bf16 inputs' source),
+ * target [n][3] = (u - u_a, v - v_a, (t - t_c)/scale) (see bbrt_mlp_features) and valid (stage 1 hit and
+ * Newton converged). */
+int bbrt_mlp_sample_pairs(bbrt_scene scene, uint64_t seed, int64_t first, int64_t n, int32_t* patch, double* origin,
+                          double* dir, float* x, float* target, int8_t* valid);
+/* Features of given (patch, ray) pairs: x [n][51] and base [n][3] = (t_c, u_a, v_a), the ray parameter of
+ * the point closest to the patch centre and the clamped tangent-plane estimate of (u, v) that the network
+ * corrects (seed = (u_a + y0, v_a + y1, t_c + scale y2)); frame [n_patches][24] (may be NULL): centre, e1,
+ * e2, normal, scale, pad[3], the tangent-plane rows a_u and a_v, pad[2]. */
+int bbrt_mlp_features(bbrt_scene scene, const int32_t* patch, const double* origin, const double* dir, int64_t n,
+                      float* x, double* base, double* frame);
+/* Attaches a snapshot of the network to the scene (per-patch frames and the
+ * patch half of layer 1 are precomputed); BBRT_SEED_LEARNED then uses it. */
+int bbrt_scene_set_mlp(bbrt_scene scene, bbrt_mlp mlp);
+/* Seed A/B on sampled pairs (indices first .. first+n-1, same draw as
+ * training): for each pair, stage 1, then converged Newton from the triangle
+ * seed and from the learned seed.  Per pair: status [2][n] (1 hit, 0 miss,
+ * -kind error), iters [2][n], accepted [n] (the learned result passes the
+ * acceptance rule), seed_err [2][n] (max |du|, |dv| of each seed to the
+ * triangle-seeded root), point_diff [n] (|p_learned - p_triangle|). */
+int bbrt_mlp_seed_ab(bbrt_scene scene, uint64_t seed, int64_t first, int64_t n, int8_t* status, uint8_t* iters,
+                     uint8_t* accepted, float* seed_err, double* point_diff);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* bbrt_last_error(void);

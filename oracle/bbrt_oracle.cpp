@@ -311,7 +311,7 @@ struct Counters {
 };
 
 // Result of one patch intersection: status 1 = hit, 0 = miss, <0 = -ErrKind.
-struct PatchHit { int status; double t; Vec3 point; Vec3 normal; int iters; };
+struct PatchHit { int status; double t; Vec3 point; Vec3 normal; int iters; double u = 0.0, v = 0.0; };
 
 // SurfacePatch.hpp:593-660 (+ the `converged` polish, which has no counterpart)
 PatchHit newton_refine(const Patch& P, const Vec3& o, const Vec3& d, double u0, double v0, double t0, int mode,
@@ -364,7 +364,7 @@ PatchHit newton_refine(const Patch& P, const Vec3& o, const Vec3& d, double u0, 
     C.newton_iters += iters;
     basis1d(P.un.data(), P.uw.data(), im, u, lu);
     basis1d(P.vn.data(), P.vw.data(), jm, v, lv);
-    r.status = 1; r.t = t; r.iters = iters;
+    r.status = 1; r.t = t; r.iters = iters; r.u = u; r.v = v;
     r.point = { eval_field(lu, lv, P.f[0].data(), im, jm), eval_field(lu, lv, P.f[1].data(), im, jm),
                 eval_field(lu, lv, P.f[2].data(), im, jm) };
     r.normal = vnormalize({ eval_field(lu, lv, P.f[9].data(), im, jm), eval_field(lu, lv, P.f[10].data(), im, jm),
@@ -755,6 +755,17 @@ int orc_patch_intersect(void* s, int64_t k, const double* o, const double* d, in
                                  mode, C);
     out[0] = h.t; out[1] = h.point.x; out[2] = h.point.y; out[3] = h.point.z;
     out[4] = h.normal.x; out[5] = h.normal.y; out[6] = h.normal.z;
+    *iters = h.iters;
+    return h.status;
+}
+
+// orc_patch_intersect with the converged parameters: out[7] = u, out[8] = v.
+int orc_patch_intersect_uv(void* s, int64_t k, const double* o, const double* d, int mode, double* out, int* iters) {
+    Counters C;
+    PatchHit h = patch_intersect(static_cast<Scene*>(s)->patches[k], { o[0], o[1], o[2] }, { d[0], d[1], d[2] },
+                                 mode, C);
+    out[0] = h.t; out[1] = h.point.x; out[2] = h.point.y; out[3] = h.point.z;
+    out[4] = h.normal.x; out[5] = h.normal.y; out[6] = h.normal.z; out[7] = h.u; out[8] = h.v;
     *iters = h.iters;
     return h.status;
 }

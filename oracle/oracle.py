@@ -57,6 +57,8 @@ def lib():
         L.orc_scene_coarse.argtypes = [C.c_void_p, C.c_int64, _d, _d, _d]
         L.orc_patch_intersect.restype = C.c_int
         L.orc_patch_intersect.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_int, _d, C.POINTER(C.c_int)]
+        L.orc_patch_intersect_uv.restype = C.c_int
+        L.orc_patch_intersect_uv.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_int, _d, C.POINTER(C.c_int)]
         L.orc_newton.restype = C.c_int
         L.orc_newton.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int, _d,
                                  C.POINTER(C.c_int)]
@@ -220,6 +222,13 @@ class OracleScene:
         st = lib().orc_patch_intersect(self.h, C.c_int64(k), ptr(_v(o), _d), ptr(_v(d), _d), C.c_int(mode),
                                        ptr(out, _d), C.byref(it))
         return st, out[0], out[1:4], out[4:7], it.value
+
+    def patch_intersect_uv(self, k, o, d, mode=0):
+        """(status, t, point, normal, iters, u, v) of SurfacePatch::intersect."""
+        out = np.zeros(9); it = C.c_int(0)
+        st = lib().orc_patch_intersect_uv(self.h, C.c_int64(k), ptr(_v(o), _d), ptr(_v(d), _d), C.c_int(mode),
+                                          ptr(out, _d), C.byref(it))
+        return st, out[0], out[1:4], out[4:7], it.value, out[7], out[8]
 
     def newton(self, k, o, d, u0, v0, t0, mode=0):
         out = np.zeros(7); it = C.c_int(0)
