@@ -18,7 +18,7 @@ NEWTON_REF, NEWTON_CONVERGED = 0, 1
 SEED_TRIANGLE, SEED_LEARNED = 0, 1
 ESCAPED, TRAPPED, ERROR = 0, 1, 2
 ERR_NONE, ERR_SINGULAR, ERR_NOCONV, ERR_DOMAIN, ERR_BACKFACE, ERR_ZERO_DIR, ERR_STACK = range(7)
-FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS, FLAG_WAVEFRONT = 1, 2, 4, 8
+FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS, FLAG_WAVEFRONT, FLAG_GRAPH = 1, 2, 4, 8, 16
 NCOUNTERS = 16
 CTR = dict(rays=0, bounces=1, bvh_node_tests=2, candidates=3, cull_box_tests=4, mt_tests=5, newton_calls=6,
            newton_iters=7, newton_flops=8, polish_fail=9, launches=10, seed_learned=11, seed_fallback=12,
@@ -31,7 +31,8 @@ EXPORTS = ("bbrt_scene_create", "bbrt_scene_destroy", "bbrt_scene_info", "bbrt_s
            "bbrt_scene_coarse", "bbrt_trace", "bbrt_trace_device", "bbrt_prepare_patches", "bbrt_build_bvh",
            "bbrt_last_error", "bbrt_mlp_create", "bbrt_mlp_destroy", "bbrt_mlp_get_params", "bbrt_mlp_set_params",
            "bbrt_mlp_forward", "bbrt_mlp_loss_grad", "bbrt_mlp_train", "bbrt_mlp_sample_pairs", "bbrt_mlp_features",
-           "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab")
+           "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab", "bbrt_scene_create_batch", "bbrt_scene_batch_info",
+           "bbrt_trace_batch", "bbrt_trace_batch_device", "bbrt_autotune")
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -49,7 +50,12 @@ class PatchSet(C.Structure):
 class Params(C.Structure):
     _fields_ = [("newton_mode", C.c_int32), ("seed_mode", C.c_int32), ("max_bounces", C.c_int32),
                 ("record_k", C.c_int32), ("flags", C.c_uint32), ("block", C.c_int32), ("waves_per_eu", C.c_int32),
-                ("newton_batch", C.c_int32)]
+                ("grid_fraction", C.c_float), ("newton_batch", C.c_int32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("block", C.c_int32), ("waves_per_eu", C.c_int32), ("newton_batch", C.c_int32),
+                ("grid_fraction", C.c_float), ("ms", C.c_double)]
 
 
 class Outputs(C.Structure):
@@ -110,6 +116,14 @@ def lib():
         L.bbrt_mlp_features.argtypes = [C.c_void_p, _i32, _d, _d, C.c_int64, _f, _d, _d]
         L.bbrt_scene_set_mlp.argtypes = [C.c_void_p, C.c_void_p]
         L.bbrt_mlp_seed_ab.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, _i8, _u8, _u8, _f, _d]
+        L.bbrt_scene_create_batch.argtypes = [C.POINTER(PatchSet), C.c_int32, C.POINTER(C.c_void_p)]
+        L.bbrt_scene_batch_info.argtypes = [C.c_void_p, _i32, _i64, _i32, _d]
+        L.bbrt_trace_batch.argtypes = [C.c_void_p, _d, _d, _i64, C.POINTER(Params), C.POINTER(Outputs), _i64]
+        L.bbrt_trace_batch_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.POINTER(Params),
+                                              C.POINTER(Outputs), _i64, C.c_void_p, _d]
+        L.bbrt_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.c_int64, C.POINTER(Params),
+                                    C.POINTER(Outputs), C.POINTER(Params), C.POINTER(TuneEntry), C.c_int32, _i32,
+                                    C.c_void_p]
         _LIB = L
     return _LIB
 

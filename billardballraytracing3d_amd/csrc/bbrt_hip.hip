@@ -30,7 +30,9 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <atomic>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "mlp_device.h"
@@ -45,7 +47,7 @@ namespace {
 // (BVH.hpp:91-97 with best_t = inf), so rays that miss the whole scene leave
 // here as Escaped after one node test.  The rest enter the live list.
 __global__ void k_init(const double* __restrict__ origins, const double* __restrict__ dirs, int64_t n,
-                       const BvhNode* __restrict__ bvh, RayState R, Outs O, int max_bounces, int32_t* live_out,
+                       DevScene S, RayState R, Outs O, int max_bounces, int32_t* live_out,
                        int32_t* n_out, unsigned long long* ctr) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
@@ -72,7 +74,7 @@ __global__ void k_init(const double* __restrict__ origins, const double* __restr
                 if (O.status) O.status[g] = bbrt::kTrapped;
                 if (O.bounces) O.bounces[g] = 0;
                 write_final(O, r, o, d);
-            } else if (!box_static(o, inv, bvh[0], tm)) {
+            } else if (!box_static(o, inv, S.bvh[S.scenes[ray_scene_of(S, g)].root], tm)) {
                 root_tests = 1;
                 if (O.status) O.status[g] = bbrt::kEscaped;
                 if (O.bounces) O.bounces[g] = 0;
@@ -130,6 +132,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
     Counters c = {};
     Counters c0 = {};
     int32_t r = -1;   // -1: wants a ray, -2: list exhausted
+    int scn = 0;      // the ray's sub-scene
     int b = 0, phase = 0;
     V3 o = { 0, 0, 0 }, d = { 0, 0, 0 };
     double inv[3] = { 0, 0, 0 };
@@ -149,15 +152,16 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
 
     // bounce set-up: RayTracingDriver.hpp:187 and the root of BVH.hpp:89
     auto start_bounce = [&](bool root_live) {
-        cull = cull_ok(S, o, d, flags);
+        cull = cull_ok(S, scn, o, d, flags);
         r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
         inv[0] = 1.0 / d.x; inv[1] = 1.0 / d.y; inv[2] = 1.0 / d.z;
         best_t = INFINITY;
         best_obj = -1; err_obj = -1; err = 0; sp = 0;
-        const BvhNode root = S.bvh[0];
+        const int32_t ri = S.scenes[scn].root;
+        const BvhNode root = S.bvh[ri];
         double tm = -INFINITY;
         const bool ok = root_live ? true : box_static(o, inv, root, tm);
-        cur = make_ent(0, root, ok, tm);
+        cur = make_ent(ri, root, ok, tm);
         phase = kPhTrav;
         if constexpr (WORK) c0 = c;
     };
@@ -173,6 +177,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 const int qq = base + (int)__popcll(want & ((1ull << lane) - 1ull));
                 if (qq < n_live) {
                     r = live[qq];
+                    scn = ray_scene_of(S, r);
                     o = { R.o[3 * r], R.o[3 * r + 1], R.o[3 * r + 2] };
                     d = { R.d[3 * r], R.d[3 * r + 1], R.d[3 * r + 2] };
                     b = 0;
@@ -278,7 +283,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         }
 
         if (r >= 0 && phase == kPhDone) {
-            const bool alive = finish_bounce(S, O, r, o, d, b, max_bounces, record_k, flags, err, err_obj, best,
+            const bool alive = finish_bounce(S, O, r, scn, o, d, b, max_bounces, record_k, flags, err, err_obj, best,
                                              best_obj, c);
             if constexpr (WORK) add_ray_work(O, r, c, c0);
             if (alive) { ++b; start_bounce(false); }
@@ -331,9 +336,26 @@ __global__ void k_zero_i32(int32_t* p, int n) {
 
 namespace {
 
+// Per-ray sub-scene ids from the sub-scene ray ranges (binary search).
+__global__ void k_ray_scene(const int64_t* __restrict__ begin, int n_scenes, int64_t n, int32_t* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int lo = 0, hi = n_scenes - 1;   // last sub-scene whose range starts at or before g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    out[g] = lo;
+}
+
 void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, const bbrt_params* p,
-                const bbrt_outputs* out, int64_t* counters, hipStream_t st, double* kernel_ms, bool pad_records) {
+                const bbrt_outputs* out, int64_t* counters, hipStream_t st, double* kernel_ms, bool pad_records,
+                const int32_t* d_ray_scene) {
     if (!p) throw std::runtime_error("bbrt_trace: params is NULL");
+    if (s->n_scenes > 1 && !d_ray_scene)
+        throw std::runtime_error("bbrt_trace: this scene holds several sub-scenes; use bbrt_trace_batch");
+    DevScene dev = s->dev;
+    dev.ray_scene = d_ray_scene;
     if (p->max_bounces < 0 || p->record_k < 0) throw std::runtime_error("bbrt_trace: negative max_bounces or record_k");
     if (p->newton_mode != BBRT_NEWTON_REF && p->newton_mode != BBRT_NEWTON_CONVERGED)
         throw std::runtime_error("bbrt_trace: unknown newton_mode");
@@ -386,53 +408,97 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             s->kev.push_back(e);
         }
     }
-    check(hipEventRecord(s->ev[0], st), "hipEventRecord");
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cnt, 4);
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, reinterpret_cast<int32_t*>(ctr), 2 * BBRT_NCOUNTERS);
-    if (pad_records && p->record_k > 0) {
-        const int64_t total = n * (int64_t)p->record_k;
-        const unsigned pg = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
-        hipLaunchKernelGGL(k_pad_records, dim3(pg), dim3(256), 0, st, O, total);
+    const bool graph = (p->flags & BBRT_FLAG_GRAPH) != 0;
+    if (graph && wavefront) throw std::runtime_error("bbrt_trace: BBRT_FLAG_GRAPH needs the persistent schedule");
+    // the persistent kernel and its grid (host-side choices, fixed in a captured graph)
+    decltype(&k_persist<2, false, false>) kern = nullptr;
+    unsigned grid = 1;
+    const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
+    const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch;
+    if (!wavefront && p->max_bounces > 0) {
+        const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : kDefaultWavesPerEu;
+        const bool work = O.ray_work != nullptr;
+        if (learned) {
+            switch (wpe) {
+                case 2: kern = &k_persist<2, false, true>; break;
+                case 3: kern = &k_persist<3, false, true>; break;
+                case 4: kern = &k_persist<4, false, true>; break;
+                default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
+            }
+        } else {
+            switch (wpe) {
+                case 2: kern = work ? &k_persist<2, true, false> : &k_persist<2, false, false>; break;
+                case 3: kern = work ? &k_persist<3, true, false> : &k_persist<3, false, false>; break;
+                case 4: kern = work ? &k_persist<4, true, false> : &k_persist<4, false, false>; break;
+                default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
+            }
+        }
+        int per_cu = 0;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0), "occupancy");
+        int64_t resident = (int64_t)std::max(1, per_cu) * s->n_cu;
+        if (p->grid_fraction > 0.0f && p->grid_fraction < 1.0f)
+            resident = std::max<int64_t>(1, (int64_t)((double)resident * p->grid_fraction));
+        grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, resident));
     }
     const int64_t grid0 = (n + 255) / 256;
-    hipLaunchKernelGGL(k_init, dim3((unsigned)grid0), dim3(256), 0, st, d_o, d_d, n, s->dev.bvh, R, O,
-                       p->max_bounces, live[0], cnt, ctr);
-    check(hipGetLastError(), "k_init launch");
-    int launches = 0;
-    if (!wavefront) {
-        if (p->max_bounces > 0) {
-            const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : kDefaultWavesPerEu;
-            decltype(&k_persist<2, false, false>) kern = nullptr;
-            const bool work = O.ray_work != nullptr;
-            if (learned) {
-                switch (wpe) {
-                    case 2: kern = &k_persist<2, false, true>; break;
-                    case 3: kern = &k_persist<3, false, true>; break;
-                    case 4: kern = &k_persist<4, false, true>; break;
-                    default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
-                }
-            } else {
-                switch (wpe) {
-                    case 2: kern = work ? &k_persist<2, true, false> : &k_persist<2, false, false>; break;
-                    case 3: kern = work ? &k_persist<3, true, false> : &k_persist<3, false, false>; break;
-                    case 4: kern = work ? &k_persist<4, true, false> : &k_persist<4, false, false>; break;
-                    default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
-                }
-            }
-            const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
-            int per_cu = 0;
-            check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0), "occupancy");
-            const int64_t resident = (int64_t)std::max(1, per_cu) * s->n_cu;
-            const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, resident));
-            if (timing) check(hipEventRecord(s->kev[0], st), "hipEventRecord");
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, s->dev, R, live[0], cnt, cnt + 2, O,
-                               p->max_bounces, p->record_k, p->newton_mode, p->flags, ctr, MD,
-                               p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch);
-            if (timing) check(hipEventRecord(s->kev[1], st), "hipEventRecord");
-            check(hipGetLastError(), "k_persist launch");
-            launches = 1;
+    // zero the counters, optional record padding, k_init, the persistent kernel
+    auto enqueue = [&](hipStream_t q, bool events) {
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, q, cnt, 4);
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, q, reinterpret_cast<int32_t*>(ctr), 2 * BBRT_NCOUNTERS);
+        if (pad_records && p->record_k > 0) {
+            const int64_t total = n * (int64_t)p->record_k;
+            const unsigned pg = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+            hipLaunchKernelGGL(k_pad_records, dim3(pg), dim3(256), 0, q, O, total);
         }
+        hipLaunchKernelGGL(k_init, dim3((unsigned)grid0), dim3(256), 0, q, d_o, d_d, n, dev, R, O,
+                           p->max_bounces, live[0], cnt, ctr);
+        if (kern) {
+            if (events) check(hipEventRecord(s->kev[0], q), "hipEventRecord");
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, q, dev, R, live[0], cnt, cnt + 2, O,
+                               p->max_bounces, p->record_k, p->newton_mode, p->flags, ctr, MD, nbatch);
+            if (events) check(hipEventRecord(s->kev[1], q), "hipEventRecord");
+        }
+    };
+    int launches = 0;
+    bool graph_timed = false;
+    if (graph) {
+        // One hipGraph per scene handle, captured on the handle's own stream and
+        // replayed while every argument is the same.
+        GraphKey key{};
+        key.d_o = d_o; key.d_d = d_d; key.n = n; key.ray_scene = d_ray_scene; key.params = *p;
+        key.outs = O; key.scratch = base; key.grid = grid; key.block = block; key.kern = (const void*)kern;
+        key.pad = pad_records;
+        hipStream_t gs = s->gstream;
+        if (st != gs) {
+            check(hipEventRecord(s->ev_dep, st), "hipEventRecord");
+            check(hipStreamWaitEvent(gs, s->ev_dep, 0), "hipStreamWaitEvent");
+        }
+        if (!s->gexec || std::memcmp(&key, &s->gkey, sizeof(GraphKey)) != 0) {
+            if (s->gexec) { check(hipGraphExecDestroy(s->gexec), "hipGraphExecDestroy"); s->gexec = nullptr; }
+            hipGraph_t g = nullptr;
+            check(hipStreamBeginCapture(gs, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+            enqueue(gs, false);
+            check(hipStreamEndCapture(gs, &g), "hipStreamEndCapture");
+            const hipError_t ie = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            check(ie, "hipGraphInstantiate");
+            s->gkey = key;
+            s->graph_captures++;
+        }
+        check(hipEventRecord(s->ev[0], gs), "hipEventRecord");
+        check(hipGraphLaunch(s->gexec, gs), "hipGraphLaunch");
+        st = gs;
+        launches = kern ? 1 : 0;
+        graph_timed = true;
+    } else if (!wavefront) {
+        check(hipEventRecord(s->ev[0], st), "hipEventRecord");
+        enqueue(st, timing);
+        check(hipGetLastError(), "bounce kernel launch");
+        launches = kern ? 1 : 0;
     } else {
+        check(hipEventRecord(s->ev[0], st), "hipEventRecord");
+        enqueue(st, false);
+        check(hipGetLastError(), "k_init launch");
         int cur = 0;
         for (int b = 0; b < p->max_bounces; ++b) {
             check(hipMemcpyAsync(s->h_count, cnt + cur, sizeof(int32_t), hipMemcpyDeviceToHost, st), "count D2H");
@@ -440,9 +506,9 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             const int32_t n_live = *s->h_count;
             if (n_live <= 0) break;
             hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, cnt + (cur ^ 1), 1);
-            const unsigned grid = (unsigned)((n_live + block - 1) / block);
+            const unsigned gridb = (unsigned)((n_live + block - 1) / block);
             if (timing) check(hipEventRecord(s->kev[2 * launches], st), "hipEventRecord");
-            hipLaunchKernelGGL(k_bounce, dim3(grid), dim3(block), 0, st, s->dev, R, live[cur], cnt + cur,
+            hipLaunchKernelGGL(k_bounce, dim3(gridb), dim3(block), 0, st, dev, R, live[cur], cnt + cur,
                                live[cur ^ 1], cnt + (cur ^ 1), O, b, p->max_bounces, p->record_k, p->newton_mode,
                                p->flags, ctr);
             if (timing) check(hipEventRecord(s->kev[2 * launches + 1], st), "hipEventRecord");
@@ -460,7 +526,11 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     if (counters) std::memcpy(counters, local_ctr, sizeof(local_ctr));
     if (kernel_ms) {
         double ksum = 0.0;
-        if (timing)
+        if (graph_timed) {
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]), "hipEventElapsedTime");
+            ksum = ms;   // the whole graph: zero fills, k_init and the persistent kernel
+        } else if (timing)
             for (int q = 0; q < launches; ++q) {
                 float ms = 0.0f;
                 check(hipEventElapsedTime(&ms, s->kev[2 * q], s->kev[2 * q + 1]), "hipEventElapsedTime");
@@ -471,6 +541,168 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         kernel_ms[0] = ksum; kernel_ms[1] = launches; kernel_ms[2] = total;
     }
 }
+
+// Builds every sub-scene on the host (in parallel over sub-scenes when there
+// are many) and uploads them as one device scene: patches, coarse data and
+// culling boxes concatenated in sub-scene order, each sub-scene's BVH appended
+// with its node, leaf and patch indices offset to global ones.
+int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, const char* who) {
+    *out = nullptr;
+    bbrt_scene s = nullptr;
+    try {
+        int ndev = 0;
+        check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (ndev <= 0) throw std::runtime_error(std::string(who) + ": no HIP device");
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<bbrt::HostScene> H((size_t)n_sets);
+        if (n_sets == 1) {
+            bbrt::build_host_scene(sets[0], H[0], (int)hw);
+        } else {
+            // sub-scenes spread over the host threads, each built single-threaded
+            std::atomic<int32_t> next{ 0 };
+            std::vector<std::string> errs((size_t)n_sets);
+            std::vector<std::thread> pool;
+            for (unsigned t = 0; t < hw; ++t)
+                pool.emplace_back([&]() {
+                    for (int32_t i = next++; i < n_sets; i = next++) {
+                        try { bbrt::build_host_scene(sets[i], H[i], 1); }
+                        catch (const std::exception& e) { errs[i] = e.what(); }
+                    }
+                });
+            for (auto& th : pool) th.join();
+            for (int32_t i = 0; i < n_sets; ++i)
+                if (!errs[i].empty()) throw std::runtime_error(std::string(who) + ": sub-scene " + std::to_string(i) + ": " + errs[i]);
+        }
+        // totals and offsets
+        int64_t n_patch = 0, n_node = 0, n_field = 0, n_bvh = 0, n_leaf = 0;
+        for (const auto& h : H) { n_patch += h.n; n_node += (int64_t)h.nodes.size(); n_field += (int64_t)h.fields.size();
+                                  n_bvh += (int64_t)h.bvh.size(); n_leaf += (int64_t)h.leaf.size(); }
+        if (n_patch >= (int64_t)1 << 27 || n_bvh >= (int64_t)1 << 28)
+            throw std::runtime_error(std::string(who) + ": too many patches or BVH nodes for one scene handle");
+        s = new bbrt_scene_s();
+        check(hipGetDevice(&s->device), "hipGetDevice");
+        check(hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, s->device), "CU count");
+        check(hipStreamCreate(&s->stream), "hipStreamCreate");
+        check(hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreateWithFlags(&s->ev_dep, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventCreate(&s->ev[0]), "hipEventCreate");
+        check(hipEventCreate(&s->ev[1]), "hipEventCreate");
+        check(hipHostMalloc(reinterpret_cast<void**>(&s->h_count), sizeof(int32_t) * 64, hipHostMallocDefault),
+              "hipHostMalloc");
+        std::vector<PatchMeta> meta;
+        meta.reserve((size_t)n_patch);
+        std::vector<BvhNode> bvh;
+        bvh.reserve((size_t)n_bvh);
+        std::vector<int32_t> leaf;
+        leaf.reserve((size_t)n_leaf);
+        s->n_scenes = n_sets;
+        s->patch_begin.assign(1, 0);
+        int64_t po = 0, no = 0, fo = 0;
+        for (int32_t i = 0; i < n_sets; ++i) {
+            const auto& h = H[i];
+            const int32_t bo = (int32_t)bvh.size(), lo = (int32_t)leaf.size();
+            for (PatchMeta m : h.meta) { m.node_off += no; m.field_off += fo; meta.push_back(m); }
+            for (BvhNode nd : h.bvh) {
+                if (nd.right >= 0) nd.right += bo; else nd.start += lo;
+                bvh.push_back(nd);
+            }
+            for (int32_t o : h.leaf) leaf.push_back(o + (int32_t)po);
+            bbrt::SceneInfo si{};
+            si.root = bo; si.patch_begin = (int32_t)po; si.diag = h.diag; si.cull_limit = 8.0 * h.scale; si.pad = 0.0;
+            s->scenes.push_back(si);
+            po += h.n; no += (int64_t)h.nodes.size(); fo += (int64_t)h.fields.size();
+            s->patch_begin.push_back(po);
+        }
+        for (const BvhNode& nd : bvh)
+            if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
+                throw std::runtime_error(std::string(who) + ": BVH exceeds the packed node limits");
+        // big arrays go straight from each sub-scene to its slice of the device array
+        auto upload_parts = [&](auto member) {
+            using T = typename std::decay_t<decltype(H[0].*member)>::value_type;
+            size_t total = 0;
+            for (const auto& h : H) total += (h.*member).size();
+            T* d = nullptr;
+            check(hipMalloc(&d, std::max<size_t>(1, total) * sizeof(T)), "hipMalloc(scene)");
+            s->allocs.push_back(d);
+            s->bytes += (int64_t)(std::max<size_t>(1, total) * sizeof(T));
+            size_t at = 0;
+            for (auto& h : H) {
+                auto& v = h.*member;
+                if (!v.empty()) check(hipMemcpy(d + at, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(scene)");
+                at += v.size();
+                std::vector<T>().swap(v);   // release host memory as it goes
+            }
+            return static_cast<const T*>(d);
+        };
+        DevScene& D = s->dev;
+        D.meta = dev_upload(meta, s->allocs, s->bytes);
+        D.nodes = upload_parts(&bbrt::HostScene::nodes);
+        D.fields = upload_parts(&bbrt::HostScene::fields);
+        D.cxyz = upload_parts(&bbrt::HostScene::cxyz);
+        D.cell = upload_parts(&bbrt::HostScene::cell);
+        D.box_cell = upload_parts(&bbrt::HostScene::box_cell);
+        D.box_tile = upload_parts(&bbrt::HostScene::box_tile);
+        D.box_block = upload_parts(&bbrt::HostScene::box_block);
+        D.box_super = upload_parts(&bbrt::HostScene::box_super);
+        D.box_patch = upload_parts(&bbrt::HostScene::box_patch);
+        D.bvh = dev_upload(bvh, s->allocs, s->bytes);
+        D.leaf = dev_upload(leaf, s->allocs, s->bytes);
+        s->d_scenes = const_cast<bbrt::SceneInfo*>(dev_upload(s->scenes, s->allocs, s->bytes));
+        D.scenes = s->d_scenes;
+        D.ray_scene = nullptr;
+        s->n_patches = n_patch;
+        for (auto& h : H) {
+            s->pbox.insert(s->pbox.end(), h.pbox.begin(), h.pbox.end());
+            s->pcent.insert(s->pcent.end(), h.pcent.begin(), h.pcent.end());
+            for (int64_t k = 0; k < h.n; ++k) s->max_sag.push_back(h.meta[k].max_sag);
+        }
+        s->bvh = std::move(bvh);
+        s->leaf = std::move(leaf);
+        s->diag = H[0].diag;
+        s->max_patch_sag = 0.0;
+        for (const auto& h : H) s->max_patch_sag = std::max(s->max_patch_sag, h.max_patch_sag);
+        *out = s;
+        return 0;
+    } catch (const std::exception& e) {
+        delete s;
+        return fail(e);
+    }
+}
+
+// Device per-ray sub-scene ids for the ray ranges `begin` (cached while they do not change).
+const int32_t* ray_scene_ids(bbrt_scene s, const int64_t* begin, hipStream_t st) {
+    const int64_t n = begin[s->n_scenes];
+    std::vector<int64_t> key(begin, begin + s->n_scenes + 1);
+    if (key == s->ray_scene_key && s->d_ray_scene) return s->d_ray_scene;
+    if (n > s->ray_scene_cap) {
+        if (s->d_ray_scene) check(hipFree(s->d_ray_scene), "hipFree");
+        s->d_ray_scene = nullptr;
+        s->ray_scene_cap = 0;
+        check(hipMalloc(&s->d_ray_scene, std::max<int64_t>(1, n) * sizeof(int32_t)), "hipMalloc(ray scene ids)");
+        s->ray_scene_cap = n;
+    }
+    int64_t* d_begin = nullptr;
+    check(hipMalloc(&d_begin, key.size() * sizeof(int64_t)), "hipMalloc");
+    check(hipMemcpyAsync(d_begin, key.data(), key.size() * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+    if (n > 0)
+        hipLaunchKernelGGL(k_ray_scene, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_begin, s->n_scenes, n,
+                           s->d_ray_scene);
+    check(hipGetLastError(), "k_ray_scene");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    check(hipFree(d_begin), "hipFree");
+    s->ray_scene_key = std::move(key);
+    return s->d_ray_scene;
+}
+
+void check_ranges(bbrt_scene s, const int64_t* begin) {
+    if (!begin) throw std::runtime_error("bbrt_trace_batch: ray_begin is NULL");
+    if (begin[0] != 0) throw std::runtime_error("bbrt_trace_batch: ray_begin[0] must be 0");
+    for (int32_t i = 0; i < s->n_scenes; ++i)
+        if (begin[i + 1] < begin[i]) throw std::runtime_error("bbrt_trace_batch: ray_begin must not decrease");
+}
+
+int trace_host(bbrt_scene s, const double* origins, const double* dirs, int64_t n, const int64_t* ray_begin,
+               const bbrt_params* p, const bbrt_outputs* out, int64_t* counters, const char* who);
 
 }  // namespace
 
@@ -526,56 +758,24 @@ int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int
 
 int bbrt_scene_create(const bbrt_patch_set* ps, bbrt_scene* out) {
     if (!ps || !out) { g_last_error = "bbrt_scene_create: NULL argument"; return -1; }
-    *out = nullptr;
-    bbrt_scene s = nullptr;
-    try {
-        int ndev = 0;
-        check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
-        if (ndev <= 0) throw std::runtime_error("bbrt_scene_create: no HIP device");
-        bbrt::HostScene H;
-        const unsigned hw = std::thread::hardware_concurrency();
-        bbrt::build_host_scene(*ps, H, (int)std::min(16u, std::max(1u, hw)));
-        s = new bbrt_scene_s();
-        check(hipGetDevice(&s->device), "hipGetDevice");
-        check(hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, s->device), "CU count");
-        for (const BvhNode& nd : H.bvh)
-            if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
-                throw std::runtime_error("bbrt_scene_create: BVH exceeds the packed node limits");
-        check(hipStreamCreate(&s->stream), "hipStreamCreate");
-        check(hipEventCreate(&s->ev[0]), "hipEventCreate");
-        check(hipEventCreate(&s->ev[1]), "hipEventCreate");
-        check(hipHostMalloc(reinterpret_cast<void**>(&s->h_count), sizeof(int32_t) * 64, hipHostMallocDefault),
-              "hipHostMalloc");
-        DevScene& D = s->dev;
-        D.meta = dev_upload(H.meta, s->allocs, s->bytes);
-        D.nodes = dev_upload(H.nodes, s->allocs, s->bytes);
-        D.fields = dev_upload(H.fields, s->allocs, s->bytes);
-        D.cxyz = dev_upload(H.cxyz, s->allocs, s->bytes);
-        D.cell = dev_upload(H.cell, s->allocs, s->bytes);
-        D.box_cell = dev_upload(H.box_cell, s->allocs, s->bytes);
-        D.box_tile = dev_upload(H.box_tile, s->allocs, s->bytes);
-        D.box_block = dev_upload(H.box_block, s->allocs, s->bytes);
-        D.box_super = dev_upload(H.box_super, s->allocs, s->bytes);
-        D.box_patch = dev_upload(H.box_patch, s->allocs, s->bytes);
-        D.bvh = dev_upload(H.bvh, s->allocs, s->bytes);
-        D.leaf = dev_upload(H.leaf, s->allocs, s->bytes);
-        D.diag = H.diag;
-        D.cull_limit = 8.0 * H.scale;
-        s->n_patches = H.n;
-        s->pbox = std::move(H.pbox);
-        s->pcent = std::move(H.pcent);
-        s->max_sag.resize(H.n);
-        for (int64_t k = 0; k < H.n; ++k) s->max_sag[k] = H.meta[k].max_sag;
-        s->bvh = std::move(H.bvh);
-        s->leaf = std::move(H.leaf);
-        s->diag = H.diag;
-        s->max_patch_sag = H.max_patch_sag;
-        *out = s;
-        return 0;
-    } catch (const std::exception& e) {
-        delete s;
-        return fail(e);
+    return create_scenes(ps, 1, out, "bbrt_scene_create");
+}
+
+int bbrt_scene_create_batch(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out) {
+    if (!sets || !out) { g_last_error = "bbrt_scene_create_batch: NULL argument"; return -1; }
+    if (n_sets <= 0) { g_last_error = "bbrt_scene_create_batch: no sub-scenes"; return -1; }
+    return create_scenes(sets, n_sets, out, "bbrt_scene_create_batch");
+}
+
+int bbrt_scene_batch_info(bbrt_scene s, int32_t* n_scenes, int64_t* patch_begin, int32_t* bvh_root, double* diag) {
+    if (!s) { g_last_error = "bbrt_scene_batch_info: NULL scene"; return -1; }
+    if (n_scenes) *n_scenes = s->n_scenes;
+    for (int32_t i = 0; i < s->n_scenes; ++i) {
+        if (bvh_root) bvh_root[i] = s->scenes[i].root;
+        if (diag) diag[i] = s->scenes[i].diag;
     }
+    if (patch_begin) std::memcpy(patch_begin, s->patch_begin.data(), s->patch_begin.size() * sizeof(int64_t));
+    return 0;
 }
 
 void bbrt_scene_destroy(bbrt_scene s) { delete s; }
@@ -641,7 +841,7 @@ int bbrt_trace_device(bbrt_scene s, const double* origins, const double* dirs, i
                       const bbrt_outputs* out, int64_t* counters, void* stream, double* kernel_ms) {
     if (!s) { g_last_error = "bbrt_trace_device: NULL scene"; return -1; }
     try {
-        trace_impl(s, origins, dirs, n, p, out, counters, static_cast<hipStream_t>(stream), kernel_ms, false);
+        trace_impl(s, origins, dirs, n, p, out, counters, static_cast<hipStream_t>(stream), kernel_ms, false, nullptr);
         return 0;
     } catch (const std::exception& e) {
         return fail(e);
@@ -650,13 +850,106 @@ int bbrt_trace_device(bbrt_scene s, const double* origins, const double* dirs, i
 
 int bbrt_trace(bbrt_scene s, const double* origins, const double* dirs, int64_t n, const bbrt_params* p,
                const bbrt_outputs* out, int64_t* counters) {
-    if (!s) { g_last_error = "bbrt_trace: NULL scene"; return -1; }
-    if (!p) { g_last_error = "bbrt_trace: params is NULL"; return -1; }
-    if (n > 0 && (!origins || !dirs)) { g_last_error = "bbrt_trace: NULL origins or dirs"; return -1; }
+    return trace_host(s, origins, dirs, n, nullptr, p, out, counters, "bbrt_trace");
+}
+
+int bbrt_trace_batch(bbrt_scene s, const double* origins, const double* dirs, const int64_t* ray_begin,
+                     const bbrt_params* p, const bbrt_outputs* out, int64_t* counters) {
+    if (!s || !ray_begin) { g_last_error = "bbrt_trace_batch: NULL scene or ray_begin"; return -1; }
+    return trace_host(s, origins, dirs, ray_begin[s->n_scenes], ray_begin, p, out, counters, "bbrt_trace_batch");
+}
+
+int bbrt_trace_batch_device(bbrt_scene s, const double* origins, const double* dirs, const int64_t* ray_begin,
+                            const bbrt_params* p, const bbrt_outputs* out, int64_t* counters, void* stream,
+                            double* kernel_ms) {
+    if (!s) { g_last_error = "bbrt_trace_batch_device: NULL scene"; return -1; }
+    try {
+        check_ranges(s, ray_begin);
+        check(hipSetDevice(s->device), "hipSetDevice");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+        const int32_t* ids = ray_scene_ids(s, ray_begin, st);
+        trace_impl(s, origins, dirs, ray_begin[s->n_scenes], p, out, counters, st, kernel_ms, false, ids);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int bbrt_autotune(bbrt_scene s, const double* origins, const double* dirs, const int64_t* ray_begin, int64_t n,
+                  const bbrt_params* base, const bbrt_outputs* out, bbrt_params* best, bbrt_tune_entry* table,
+                  int32_t max_entries, int32_t* n_entries, void* stream) {
+    if (!s || !base || !best) { g_last_error = "bbrt_autotune: NULL argument"; return -1; }
+    try {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+        const int32_t* ids = nullptr;
+        if (ray_begin) {
+            check_ranges(s, ray_begin);
+            ids = ray_scene_ids(s, ray_begin, st);
+            n = ray_begin[s->n_scenes];
+        }
+        if (base->flags & BBRT_FLAG_WAVEFRONT) throw std::runtime_error("bbrt_autotune: tunes the persistent schedule only");
+        int32_t count = 0;
+        auto run = [&](const bbrt_params& q) {
+            double best_ms = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {
+                double kms[3] = { 0, 0, 0 };
+                trace_impl(s, origins, dirs, n, &q, out, nullptr, st, kms, false, ids);
+                best_ms = std::min(best_ms, kms[2]);
+            }
+            if (table && count < max_entries)
+                table[count] = bbrt_tune_entry{ q.block, q.waves_per_eu, q.newton_batch, q.grid_fraction, best_ms };
+            ++count;
+            return best_ms;
+        };
+        // coordinate descent from the library defaults over occupancy, Newton batch, block size and grid size
+        bbrt_params cur = *base;
+        cur.flags &= ~(uint32_t)BBRT_FLAG_GRAPH;
+        cur.block = 256; cur.waves_per_eu = kDefaultWavesPerEu; cur.newton_batch = kDefaultNewtonBatch;
+        cur.grid_fraction = 1.0f;
+        double cur_ms = run(cur);
+        auto sweep = [&](auto setter, std::initializer_list<double> values) {
+            bbrt_params keep = cur;
+            for (double v : values) {
+                bbrt_params q = cur;
+                setter(q, v);
+                if (std::memcmp(&q, &cur, sizeof(q)) == 0) continue;
+                const double ms = run(q);
+                if (ms < cur_ms) { cur_ms = ms; keep = q; }
+            }
+            cur = keep;
+        };
+        sweep([](bbrt_params& q, double v) { q.waves_per_eu = (int)v; }, { 2, 3, 4 });
+        sweep([](bbrt_params& q, double v) { q.newton_batch = (int)v; }, { 8, 16, 32, 64 });
+        sweep([](bbrt_params& q, double v) { q.block = (int)v; }, { 128, 256 });
+        sweep([](bbrt_params& q, double v) { q.grid_fraction = (float)v; }, { 0.5, 0.75, 1.0 });
+        *best = cur;
+        best->flags = base->flags;
+        if (n_entries) *n_entries = count;
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+int trace_host(bbrt_scene s, const double* origins, const double* dirs, int64_t n, const int64_t* ray_begin,
+               const bbrt_params* p, const bbrt_outputs* out, int64_t* counters, const char* who) {
+    if (!s) { g_last_error = std::string(who) + ": NULL scene"; return -1; }
+    if (!p) { g_last_error = std::string(who) + ": params is NULL"; return -1; }
+    if (n > 0 && (!origins || !dirs)) { g_last_error = std::string(who) + ": NULL origins or dirs"; return -1; }
     std::vector<void*> tmp;
     auto cleanup = [&]() { for (void* q : tmp) (void)hipFree(q); };
     try {
         check(hipSetDevice(s->device), "hipSetDevice");
+        const int32_t* ids = nullptr;
+        if (ray_begin) {
+            check_ranges(s, ray_begin);
+            ids = ray_scene_ids(s, ray_begin, s->stream);
+        }
         const int64_t K = p->record_k;
         auto dalloc = [&](size_t bytes) -> void* {
             void* q = nullptr;
@@ -686,7 +979,7 @@ int bbrt_trace(bbrt_scene s, const double* origins, const double* dirs, int64_t 
             dout.hit_count = static_cast<int64_t*>(dalloc((size_t)s->n_patches * 8));
             check(hipMemcpy(dout.hit_count, hout.hit_count, (size_t)s->n_patches * 8, hipMemcpyHostToDevice), "H2D hit_count");
         }
-        trace_impl(s, d_o, d_d, n, p, &dout, counters, s->stream, nullptr, true);
+        trace_impl(s, d_o, d_d, n, p, &dout, counters, s->stream, nullptr, true, ids);
         auto back = [&](void* h, const void* d, size_t bytes) {
             if (h && bytes) check(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost), "D2H outputs");
         };
@@ -708,4 +1001,4 @@ int bbrt_trace(bbrt_scene s, const double* origins, const double* dirs, int64_t 
     }
 }
 
-}  // extern "C"
+}  // namespace

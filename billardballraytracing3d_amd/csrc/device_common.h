@@ -41,9 +41,11 @@ struct DevScene {
     const float* box_patch;
     const BvhNode* bvh;
     const int32_t* leaf;
-    double diag;
-    double cull_limit;   // culling is used while |origin|_inf <= cull_limit
+    const bbrt::SceneInfo* scenes;   // per sub-scene: BVH root, first patch, diagonal, culling limit
+    const int32_t* ray_scene;        // per ray: its sub-scene (NULL: every ray in sub-scene 0)
 };
+
+__device__ __forceinline__ int ray_scene_of(const DevScene& S, int64_t r) { return S.ray_scene ? S.ray_scene[r] : 0; }
 
 struct RayState {
     double* o;   // [n][3]
@@ -542,8 +544,8 @@ __device__ __forceinline__ Ent make_ent(int32_t idx, const BvhNode& nd, bool ok,
 // dead entry) so the visit order, the stack depth and the node-test count are
 // the reference's.  The near child stays in registers instead of being pushed
 // and popped at once.  Returns 0 or an error kind.
-__device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32& r32, bool cull, int mode,
-                        bool root_live, PHit& best, int32_t& best_obj, int32_t& err_obj, Counters& c) {
+__device__ int traverse(const DevScene& S, int32_t root_idx, const V3& o, const V3& d, const Ray32& r32, bool cull,
+                        int mode, bool root_live, PHit& best, int32_t& best_obj, int32_t& err_obj, Counters& c) {
     const double inv[3] = { 1.0 / d.x, 1.0 / d.y, 1.0 / d.z };
     const bool neg[3] = { inv[0] < 0.0, inv[1] < 0.0, inv[2] < 0.0 };
     double best_t = INFINITY;
@@ -553,10 +555,10 @@ __device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32
     int sp = 0;
     Ent cur;
     {
-        const BvhNode root = S.bvh[0];
+        const BvhNode root = S.bvh[root_idx];
         double tm = -INFINITY;
         const bool ok = root_live ? true : box_static(o, inv, root, tm);
-        cur = make_ent(0, root, ok, tm);
+        cur = make_ent(root_idx, root, ok, tm);
     }
     // While-while form: the inner loop walks internal nodes (cheap) until the
     // lane stands on a leaf to intersect or runs out of nodes; leaves are then
@@ -622,18 +624,18 @@ __device__ int traverse(const DevScene& S, const V3& o, const V3& d, const Ray32
     return 0;
 }
 
-// One bounce of ray r: RayTracingDriver.hpp:186-237.  Returns true while the
-// ray continues; o and d are advanced in place.
 // The end of one bounce, RayTracingDriver.hpp:190-237: error, escape, back
 // face, or record + reflect + nudge.  Returns true while the ray continues;
-// o and d are advanced in place.
-__device__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, V3& o, V3& d, int bounce, int max_bounces,
-                              int record_k, uint32_t flags, int err, int32_t err_obj, const PHit& best,
-                              int32_t best_obj, Counters& c) {
+// o and d are advanced in place.  Patch indices are recorded relative to the
+// ray's sub-scene; hit_count is indexed by the global patch index.
+__device__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, int scn, V3& o, V3& d, int bounce,
+                              int max_bounces, int record_k, uint32_t flags, int err, int32_t err_obj,
+                              const PHit& best, int32_t best_obj, Counters& c) {
+    const bbrt::SceneInfo& SI = S.scenes[scn];
     if (err) {
         if (O.status) O.status[r] = bbrt::kError;
         if (O.err_kind) O.err_kind[r] = (int8_t)err;
-        if (O.err_patch) O.err_patch[r] = err_obj;
+        if (O.err_patch) O.err_patch[r] = err_obj >= 0 ? err_obj - SI.patch_begin : err_obj;
         if (O.bounces) O.bounces[r] = bounce;
         write_final(O, r, o, d);
         return false;
@@ -647,14 +649,14 @@ __device__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, V3& o
     if (dot3(d, best.n) > kBackfaceTol) {
         if (O.status) O.status[r] = bbrt::kError;
         if (O.err_kind) O.err_kind[r] = bbrt::kErrBackface;
-        if (O.err_patch) O.err_patch[r] = best_obj;
+        if (O.err_patch) O.err_patch[r] = best_obj - SI.patch_begin;
         if (O.bounces) O.bounces[r] = bounce;
         write_final(O, r, o, d);
         return false;
     }
     if (bounce < record_k) {
         const size_t at = (size_t)r * record_k + bounce;
-        if (O.hit_patch) O.hit_patch[at] = best_obj;
+        if (O.hit_patch) O.hit_patch[at] = best_obj - SI.patch_begin;
         if (O.newton_iters) O.newton_iters[at] = (uint8_t)best.iters;
         if (O.hit_xyz) { O.hit_xyz[3 * at] = best.p.x; O.hit_xyz[3 * at + 1] = best.p.y; O.hit_xyz[3 * at + 2] = best.p.z; }
     }
@@ -665,7 +667,7 @@ __device__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, V3& o
     const V3& nn = best.n;
     const double dn = d.x * nn.x + d.y * nn.y + d.z * nn.z;
     d = { d.x - 2.0 * dn * nn.x, d.y - 2.0 * dn * nn.y, d.z - 2.0 * dn * nn.z };
-    const double eps = smax(kSelfEpsFactor * S.diag, kSagittaClearance * S.meta[best_obj].max_sag);
+    const double eps = smax(kSelfEpsFactor * SI.diag, kSagittaClearance * S.meta[best_obj].max_sag);
     o = { best.p.x + d.x * eps, best.p.y + d.y * eps, best.p.z + d.z * eps };
     if (bounce + 1 >= max_bounces) {
         if (O.status) O.status[r] = bbrt::kTrapped;
@@ -676,11 +678,11 @@ __device__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, V3& o
     return true;
 }
 
-__device__ __forceinline__ bool cull_ok(const DevScene& S, const V3& o, const V3& d, uint32_t flags) {
+__device__ __forceinline__ bool cull_ok(const DevScene& S, int scn, const V3& o, const V3& d, uint32_t flags) {
     const bool finite = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(d.x) && isfinite(d.y) &&
                         isfinite(d.z);
-    return !(flags & BBRT_FLAG_NO_CULL) && finite && fabs(o.x) <= S.cull_limit && fabs(o.y) <= S.cull_limit &&
-           fabs(o.z) <= S.cull_limit;
+    const double lim = S.scenes[scn].cull_limit;
+    return !(flags & BBRT_FLAG_NO_CULL) && finite && fabs(o.x) <= lim && fabs(o.y) <= lim && fabs(o.z) <= lim;
 }
 
 __device__ __forceinline__ void add_ray_work(const Outs& O, int32_t r, const Counters& c, const Counters& c0) {
@@ -697,12 +699,14 @@ __device__ bool bounce_once(const DevScene& S, const Outs& O, int32_t r, V3& o, 
                             int record_k, int mode, uint32_t flags, bool root_live, Counters& c) {
     Counters c0;
     if constexpr (WORK) c0 = c;
-    const bool cull = cull_ok(S, o, d, flags);
+    const int scn = ray_scene_of(S, r);
+    const bool cull = cull_ok(S, scn, o, d, flags);
     const Ray32 r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
     PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     int32_t best_obj, err_obj;
-    const int err = traverse(S, o, d, r32, cull, mode, root_live, best, best_obj, err_obj, c);
-    const bool alive = finish_bounce(S, O, r, o, d, bounce, max_bounces, record_k, flags, err, err_obj, best, best_obj, c);
+    const int err = traverse(S, S.scenes[scn].root, o, d, r32, cull, mode, root_live, best, best_obj, err_obj, c);
+    const bool alive = finish_bounce(S, O, r, scn, o, d, bounce, max_bounces, record_k, flags, err, err_obj, best,
+                                     best_obj, c);
     if constexpr (WORK) add_ray_work(O, r, c, c0);
     return alive;
 }

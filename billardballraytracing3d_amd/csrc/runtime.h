@@ -42,6 +42,21 @@ T* dev_upload(const std::vector<T>& v, std::vector<void*>& allocs, int64_t& byte
 }  // namespace
 
 
+// Everything a captured trace graph depends on; a replay needs all of it unchanged.
+struct GraphKey {
+    const double* d_o;
+    const double* d_d;
+    int64_t n;
+    const int32_t* ray_scene;
+    bbrt_params params;
+    Outs outs;
+    const void* scratch;
+    unsigned grid;
+    int block;
+    const void* kern;
+    bool pad;
+};
+
 struct bbrt_scene_s {
     int device = 0;
     int n_cu = 1;
@@ -59,6 +74,21 @@ struct bbrt_scene_s {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     int32_t* h_count = nullptr;   // pinned
+    // BBRT_FLAG_GRAPH: the captured trace and the arguments it was captured with
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev_dep = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{};
+    int64_t graph_captures = 0;
+    // sub-scenes (one for bbrt_scene_create, many for bbrt_scene_create_batch)
+    int32_t n_scenes = 1;
+    std::vector<bbrt::SceneInfo> scenes;
+    bbrt::SceneInfo* d_scenes = nullptr;
+    std::vector<int64_t> patch_begin;   // [n_scenes + 1]
+    // per-ray sub-scene ids of the last batch trace, rebuilt when the ray ranges change
+    int32_t* d_ray_scene = nullptr;
+    int64_t ray_scene_cap = 0;
+    std::vector<int64_t> ray_scene_key;
     // learned seed: per-patch frames (fp64 [n][16]) and features (fp32 [n][48]),
     // and the attached network's snapshot (bbrt_scene_set_mlp)
     double* mlp_frame = nullptr;
@@ -72,8 +102,12 @@ struct bbrt_scene_s {
 
     ~bbrt_scene_s() {
         for (void* p : allocs) (void)hipFree(p);
+        if (d_ray_scene) (void)hipFree(d_ray_scene);
         if (scratch) (void)hipFree(scratch);
         if (h_count) (void)hipHostFree(h_count);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (ev_dep) (void)hipEventDestroy(ev_dep);
+        if (gstream) (void)hipStreamDestroy(gstream);
         for (auto e : ev) if (e) (void)hipEventDestroy(e);
         for (auto e : kev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);

@@ -52,6 +52,15 @@ struct PatchMeta {
     double pad1;
 };
 
+// Per sub-scene of a scene handle (a batch holds many), 32 bytes.
+struct SceneInfo {
+    int32_t root;          // BVH root node (global node index)
+    int32_t patch_begin;   // first global patch index
+    double diag;           // scene diagonal (RayTracingDriver.hpp:386-396)
+    double cull_limit;     // culling is used while |origin|_inf <= cull_limit
+    double pad;
+};
+
 // BVH node, 64 bytes, the reference's depth-first layout (BVH.hpp:56-62).
 struct BvhNode {
     double lo[3], hi[3];

@@ -428,6 +428,59 @@ def config2(seed: int = 0, n_rays: int = 1 << 20):
     return sc, o, d, 100
 
 
+def batch_obstacles(rng: "SplitMix64", patch_lo: float, patch_hi: float) -> List[Obstacle]:
+    """One scene of the batched bounce map: 1-8 obstacles (sphere or ellipsoid, 50/50), patch counts
+    log-uniform in [patch_lo, patch_hi] snapped to an icosphere (20 f^2) or octasphere (8 f^2), order 2 or 4,
+    non-overlapping centres in [-4, 4]^3."""
+    n_obs = int(rng.integers(1, 1, 8)[0])
+    specs = []
+    for _ in range(n_obs):
+        kind = "sphere" if rng.uniform(1)[0] < 0.5 else "ellipsoid"
+        target = math.exp(rng.uniform(1, math.log(patch_lo), math.log(patch_hi))[0])
+        f_ico = max(2, int(round(math.sqrt(target / 20.0))))
+        f_oct = max(3, int(round(math.sqrt(target / 8.0))))
+        mesh, f = ("ico", f_ico) if abs(math.log(20 * f_ico ** 2 / target)) <= abs(math.log(8 * f_oct ** 2 / target)) \
+            else ("oct", f_oct)
+        order = 2 if rng.uniform(1)[0] < 0.5 else 4
+        if kind == "sphere":
+            r = rng.uniform(1, 0.5, 1.2)[0]
+            axes = np.array([r, r, r])
+        else:
+            axes = rng.uniform(3, 0.5, 1.5)
+        specs.append((kind, mesh, f, order, axes))
+    centres: List[np.ndarray] = []
+    obs = []
+    for i, (kind, mesh, f, order, axes) in enumerate(specs):
+        rad = float(np.max(axes))
+        for _ in range(10000):
+            c = rng.uniform(3, -4.0, 4.0)
+            if all(np.linalg.norm(c - c2) > rad + r2 + 0.05 for c2, r2 in centres):
+                break
+        centres.append((c, rad))
+        obs.append(Obstacle(kind, c, axes, mesh, f, order, rot_seed=int(rng.integers(1, 0, 1 << 30)[0])))
+    return obs
+
+
+def batch_config(seed: int = 0, n_scenes: int = 256, patch_lo: float = 80, patch_hi: float = 5120,
+                 rays_lo: float = 1000, rays_hi: float = 64000, cone_deg: float = 30.0):
+    """The batched bounce map: n_scenes independent scenes (batch_obstacles) with ray counts log-uniform in
+    [rays_lo, rays_hi], origins uniform on z = -8 over [-6, 6]^2, directions uniform in a cone about +z.
+
+    Returns (scenes, origins, dirs, ray_begin): the rays of scene i are ray_begin[i] .. ray_begin[i+1]-1.
+    """
+    rng = SplitMix64(0xBA7C4 + seed)
+    scenes, os_, ds_ = [], [], []
+    begin = [0]
+    for k in range(n_scenes):
+        scenes.append(build_scene(batch_obstacles(rng, patch_lo, patch_hi)))
+        nr = int(round(math.exp(rng.uniform(1, math.log(rays_lo), math.log(rays_hi))[0])))
+        o, d = cone_rays(nr, (seed << 16) + k, 6.0, -8.0, cone_deg)
+        os_.append(o); ds_.append(d)
+        begin.append(begin[-1] + nr)
+    return scenes, np.ascontiguousarray(np.concatenate(os_)), np.ascontiguousarray(np.concatenate(ds_)), \
+        np.array(begin, np.int64)
+
+
 # --------------------------------------------------------------------------
 # IFGF-RP patch files
 # --------------------------------------------------------------------------

@@ -58,14 +58,46 @@ def build_bvh(box, cent, leaf_threshold: int = 4):
     return dict(box=nb[:nn], right=r[:nn], start=s[:nn], count=c[:nn], axis=a[:nn], leaf=lo)
 
 
-class GpuScene:
-    """A scene resident on the GPU (bbrt_scene)."""
+def _params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newton_batch, grid_fraction=0.0):
+    return L.Params(newton_mode=mode, seed_mode=seed, max_bounces=max_bounces, record_k=record_k, flags=flags,
+                    block=block, waves_per_eu=waves_per_eu, grid_fraction=grid_fraction, newton_batch=newton_batch)
 
-    def __init__(self, sc):
+
+def _outputs(out_ptrs: dict):
+    return L.Outputs(*[C.c_void_p(out_ptrs.get(k) or 0) if out_ptrs.get(k) else None
+                       for k in ("status", "err_kind", "bounces", "err_patch", "hit_patch", "hit_xyz",
+                                 "newton_iters", "final_od", "hit_count", "ray_work")])
+
+
+class GpuScene:
+    """A scene resident on the GPU (bbrt_scene); GpuScene.batch(scenes) holds several sub-scenes."""
+
+    def __init__(self, sc=None):
+        self.n_scenes = 1
+        if sc is None:
+            return
         self.n_patches = sc.n_patches
         h = C.c_void_p()
         L.check(L.lib().bbrt_scene_create(C.byref(L.patch_set(sc)), C.byref(h)), "bbrt_scene_create")
         self.h = h
+        self.patch_begin = np.array([0, sc.n_patches], np.int64)
+
+    @classmethod
+    def batch(cls, scenes):
+        """Several independent scenes in one handle (bbrt_scene_create_batch)."""
+        self = cls(None)
+        sets = (L.PatchSet * len(scenes))(*[L.patch_set(sc) for sc in scenes])
+        self._keep = scenes
+        h = C.c_void_p()
+        L.check(L.lib().bbrt_scene_create_batch(sets, C.c_int32(len(scenes)), C.byref(h)), "bbrt_scene_create_batch")
+        self.h = h
+        self.n_scenes = len(scenes)
+        self.patch_begin = np.zeros(self.n_scenes + 1, np.int64)
+        ns = C.c_int32()
+        L.check(L.lib().bbrt_scene_batch_info(self.h, C.byref(ns), L.ptr(self.patch_begin, L._i64), None, None),
+                "bbrt_scene_batch_info")
+        self.n_patches = int(self.patch_begin[-1])
+        return self
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -109,7 +141,7 @@ class GpuScene:
     def trace(self, origins, dirs, max_bounces: int, mode: int = L.NEWTON_CONVERGED,
               record_k: Optional[int] = None, flags: int = 0, hit_count: bool = False, block: int = 0,
               ray_work: bool = False, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE,
-              newton_batch: int = 0) -> TraceResult:
+              newton_batch: int = 0, grid_fraction: float = 0.0, ray_begin=None) -> TraceResult:
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
         d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
         if o.shape != d.shape:
@@ -121,30 +153,65 @@ class GpuScene:
                           np.zeros((n, 6)), np.zeros(L.NCOUNTERS, np.int64),
                           np.zeros(self.n_patches, np.int64) if hit_count else None,
                           np.zeros((n, 4), np.uint32) if ray_work else None)
-        p = L.Params(mode, seed, max_bounces, K, flags, block, waves_per_eu, newton_batch)
+        p = _params(mode, seed, max_bounces, K, flags, block, waves_per_eu, newton_batch, grid_fraction)
         out = L.Outputs(L.vptr(res.status), L.vptr(res.err_kind), L.vptr(res.bounces), L.vptr(res.err_patch),
                         L.vptr(res.hit_patch), L.vptr(res.hit_xyz), L.vptr(res.iters), L.vptr(res.final_od),
                         L.vptr(res.hit_count), L.vptr(res.ray_work))
-        rc = L.lib().bbrt_trace(self.h, L.ptr(o, L._d), L.ptr(d, L._d), C.c_int64(n), C.byref(p), C.byref(out),
-                                L.ptr(res.counters, L._i64))
-        L.check(rc, "bbrt_trace")
+        if ray_begin is None:
+            rc = L.lib().bbrt_trace(self.h, L.ptr(o, L._d), L.ptr(d, L._d), C.c_int64(n), C.byref(p), C.byref(out),
+                                    L.ptr(res.counters, L._i64))
+            L.check(rc, "bbrt_trace")
+        else:
+            rb = np.ascontiguousarray(ray_begin, dtype=np.int64)
+            if rb.size != self.n_scenes + 1 or rb[-1] != n:
+                raise ValueError("ray_begin must have n_scenes + 1 entries ending at the ray count")
+            rc = L.lib().bbrt_trace_batch(self.h, L.ptr(o, L._d), L.ptr(d, L._d), L.ptr(rb, L._i64), C.byref(p),
+                                          C.byref(out), L.ptr(res.counters, L._i64))
+            L.check(rc, "bbrt_trace_batch")
         return res
 
     def trace_device(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
                      mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0, stream: int = 0,
-                     block: int = 0, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE, newton_batch: int = 0):
-        """Trace with device-resident inputs/outputs (raw device addresses). Returns (counters, kernel_ms[3])."""
-        p = L.Params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newton_batch)
-        out = L.Outputs(*[C.c_void_p(out_ptrs.get(k) or 0) if out_ptrs.get(k) else None
-                          for k in ("status", "err_kind", "bounces", "err_patch", "hit_patch", "hit_xyz",
-                                    "newton_iters", "final_od", "hit_count", "ray_work")])
+                     block: int = 0, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE, newton_batch: int = 0,
+                     grid_fraction: float = 0.0, ray_begin=None):
+        """Trace with device-resident inputs/outputs (raw device addresses). Returns (counters, kernel_ms[3]).
+        With ray_begin (host array, n_scenes + 1 entries) the rays are traced per sub-scene (bbrt_trace_batch_device)."""
+        p = _params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newton_batch, grid_fraction)
+        out = _outputs(out_ptrs)
         counters = np.zeros(L.NCOUNTERS, np.int64)
         kms = np.zeros(3)
-        rc = L.lib().bbrt_trace_device(self.h, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), C.c_int64(n),
-                                       C.byref(p), C.byref(out), L.ptr(counters, L._i64),
-                                       C.c_void_p(stream) if stream else None, L.ptr(kms, L._d))
-        L.check(rc, "bbrt_trace_device")
+        st = C.c_void_p(stream) if stream else None
+        if ray_begin is None:
+            rc = L.lib().bbrt_trace_device(self.h, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), C.c_int64(n),
+                                           C.byref(p), C.byref(out), L.ptr(counters, L._i64), st, L.ptr(kms, L._d))
+            L.check(rc, "bbrt_trace_device")
+        else:
+            rb = np.ascontiguousarray(ray_begin, dtype=np.int64)
+            rc = L.lib().bbrt_trace_batch_device(self.h, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr),
+                                                 L.ptr(rb, L._i64), C.byref(p), C.byref(out),
+                                                 L.ptr(counters, L._i64), st, L.ptr(kms, L._d))
+            L.check(rc, "bbrt_trace_batch_device")
         return counters, kms
+
+    def autotune(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
+                 mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0, stream: int = 0,
+                 seed: int = L.SEED_TRIANGLE, ray_begin=None):
+        """bbrt_autotune: returns (best launch shape as a dict, list of timed candidates)."""
+        base = _params(mode, seed, max_bounces, record_k, flags, 0, 0, 0)
+        best = L.Params()
+        table = (L.TuneEntry * 64)()
+        cnt = C.c_int32()
+        rb = None if ray_begin is None else np.ascontiguousarray(ray_begin, dtype=np.int64)
+        rc = L.lib().bbrt_autotune(self.h, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr),
+                                   L.ptr(rb, L._i64) if rb is not None else None, C.c_int64(n), C.byref(base),
+                                   C.byref(_outputs(out_ptrs)), C.byref(best), table, C.c_int32(64), C.byref(cnt),
+                                   C.c_void_p(stream) if stream else None)
+        L.check(rc, "bbrt_autotune")
+        pick = dict(block=best.block, waves_per_eu=best.waves_per_eu, newton_batch=best.newton_batch,
+                    grid_fraction=best.grid_fraction)
+        rows = [dict(block=t.block, waves_per_eu=t.waves_per_eu, newton_batch=t.newton_batch,
+                     grid_fraction=round(t.grid_fraction, 3), ms=t.ms) for t in table[:cnt.value]]
+        return pick, rows
 
     # ------------------------------------------------------------------
     # learned seed

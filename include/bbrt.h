@@ -63,8 +63,11 @@ enum { BBRT_ERR_NONE = 0, BBRT_ERR_SINGULAR = 1, BBRT_ERR_NOCONV = 2, BBRT_ERR_D
 enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute force (all 58,806 tests) */
        BBRT_FLAG_COUNT_DIRECT_HIT = 2u, /* hit_count also counts each ray's first hit (PlaneSourceDriver.hpp:101) */
        BBRT_FLAG_TIME_KERNELS = 4u,     /* record HIP events around every bounce-kernel launch */
-       BBRT_FLAG_WAVEFRONT = 8u };      /* one launch per bounce with live-ray compaction instead of the
+       BBRT_FLAG_WAVEFRONT = 8u,        /* one launch per bounce with live-ray compaction instead of the
                                            persistent kernel with a device ray queue */
+       BBRT_FLAG_GRAPH = 16u };         /* capture the trace (counter fills, k_init, persistent kernel) in a
+                                           hipGraph on the handle's stream and replay it while the arguments
+                                           are unchanged; kernel_ms[0] is then the whole graph's time */
 
 typedef struct {
     int32_t newton_mode;   /* BBRT_NEWTON_* */
@@ -75,6 +78,7 @@ typedef struct {
     int32_t block;         /* threads per block of the bounce kernel; 0 = library default */
     int32_t waves_per_eu;  /* occupancy target of the persistent kernel: 0 = library default, or 2, 3, 4
                               (the compiler keeps registers within that occupancy, spilling if needed) */
+    float grid_fraction;   /* persistent kernel: fraction of the resident workgroups launched (0 = all) */
     int32_t newton_batch;  /* persistent kernel: lanes of a wave whose stage 1 hit wait until this many can run
                               Newton (and the learned seed's network) together (1..64; 0 = library default, 64);
                               results do not depend on it */
@@ -154,6 +158,43 @@ int bbrt_trace(bbrt_scene scene, const double* origins, const double* dirs, int6
 int bbrt_trace_device(bbrt_scene scene, const double* origins, const double* dirs, int64_t n,
                       const bbrt_params* params, const bbrt_outputs* out, int64_t* counters, void* stream,
                       double* kernel_ms);
+
+/* Several independent scenes in one handle (the batched bounce map): sub-scene
+ * i is built from sets[i] exactly as bbrt_scene_create builds a scene (own
+ * BVH, diagonal and culling limit); its patches get global indices
+ * patch_begin[i] .. patch_begin[i+1]-1.  No counterpart in the reference,
+ * whose drivers trace one scene at a time (RayTracingDriver.hpp:119-150). */
+int bbrt_scene_create_batch(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out);
+/* Sub-scene count, patch_begin [n+1], BVH root node [n], diagonal [n]; any pointer may be NULL. */
+int bbrt_scene_batch_info(bbrt_scene scene, int32_t* n_scenes, int64_t* patch_begin, int32_t* bvh_root,
+                          double* diag);
+/* Traces all sub-scenes' rays in one persistent launch.  ray_begin [n_scenes+1]
+ * (host array): rays ray_begin[i] .. ray_begin[i+1]-1 belong to sub-scene i,
+ * ray_begin[0] = 0.  Outputs are per ray as for bbrt_trace; hit_patch and
+ * err_patch hold sub-scene-local patch indices, hit_count is indexed by
+ * global patch index.  bbrt_trace / bbrt_trace_device refuse a handle with
+ * more than one sub-scene. */
+int bbrt_trace_batch(bbrt_scene scene, const double* origins, const double* dirs, const int64_t* ray_begin,
+                     const bbrt_params* params, const bbrt_outputs* out, int64_t* counters);
+int bbrt_trace_batch_device(bbrt_scene scene, const double* origins, const double* dirs, const int64_t* ray_begin,
+                            const bbrt_params* params, const bbrt_outputs* out, int64_t* counters, void* stream,
+                            double* kernel_ms);
+
+/* Launch-shape autotuner for the persistent kernel.  Times the trace of the
+ * given device-resident rays (ray_begin as for bbrt_trace_batch, or NULL and n
+ * rays of a single scene; outputs in `out` are overwritten) under a coordinate
+ * descent from the library defaults: waves_per_eu {2, 3, 4}, newton_batch
+ * {8, 16, 32, 64}, block {128, 256}, grid_fraction {0.5, 0.75, 1}; each
+ * candidate is the faster of two runs.  *best = *base with the fastest shape;
+ * table (may be NULL) receives up to max_entries timed candidates in order. */
+typedef struct {
+    int32_t block, waves_per_eu, newton_batch;
+    float grid_fraction;
+    double ms;
+} bbrt_tune_entry;
+int bbrt_autotune(bbrt_scene scene, const double* origins, const double* dirs, const int64_t* ray_begin, int64_t n,
+                  const bbrt_params* base, const bbrt_outputs* out, bbrt_params* best, bbrt_tune_entry* table,
+                  int32_t max_entries, int32_t* n_entries, void* stream);
 
 /* Host-only scene preparation (no device needed): what bbrt_scene_create
  * computes per patch.  box [n][6], centroid [n][3], max_sagitta [n],
