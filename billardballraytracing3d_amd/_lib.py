@@ -19,6 +19,7 @@ SEED_TRIANGLE, SEED_LEARNED = 0, 1
 ESCAPED, TRAPPED, ERROR = 0, 1, 2
 ERR_NONE, ERR_SINGULAR, ERR_NOCONV, ERR_DOMAIN, ERR_BACKFACE, ERR_ZERO_DIR, ERR_STACK = range(7)
 FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS, FLAG_WAVEFRONT, FLAG_GRAPH = 1, 2, 4, 8, 16
+SCENE_COMPACT = 1
 NCOUNTERS = 16
 CTR = dict(rays=0, bounces=1, bvh_node_tests=2, candidates=3, cull_box_tests=4, mt_tests=5, newton_calls=6,
            newton_iters=7, newton_flops=8, polish_fail=9, launches=10, seed_learned=11, seed_fallback=12,
@@ -32,7 +33,7 @@ EXPORTS = ("bbrt_scene_create", "bbrt_scene_destroy", "bbrt_scene_info", "bbrt_s
            "bbrt_last_error", "bbrt_mlp_create", "bbrt_mlp_destroy", "bbrt_mlp_get_params", "bbrt_mlp_set_params",
            "bbrt_mlp_forward", "bbrt_mlp_loss_grad", "bbrt_mlp_train", "bbrt_mlp_sample_pairs", "bbrt_mlp_features",
            "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab", "bbrt_scene_create_batch", "bbrt_scene_batch_info",
-           "bbrt_trace_batch", "bbrt_trace_batch_device", "bbrt_autotune")
+           "bbrt_trace_batch", "bbrt_trace_batch_device", "bbrt_autotune", "bbrt_scene_create_ex")
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -118,6 +119,7 @@ def lib():
         L.bbrt_mlp_seed_ab.argtypes = [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, _i8, _u8, _u8, _f, _d]
         L.bbrt_scene_create_batch.argtypes = [C.POINTER(PatchSet), C.c_int32, C.POINTER(C.c_void_p)]
         L.bbrt_scene_batch_info.argtypes = [C.c_void_p, _i32, _i64, _i32, _d]
+        L.bbrt_scene_create_ex.argtypes = [C.POINTER(PatchSet), C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]
         L.bbrt_trace_batch.argtypes = [C.c_void_p, _d, _d, _i64, C.POINTER(Params), C.POINTER(Outputs), _i64]
         L.bbrt_trace_batch_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.POINTER(Params),
                                               C.POINTER(Outputs), _i64, C.c_void_p, _d]

@@ -240,7 +240,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
             obj = S.leaf[leaf_start + q];
             have_h = true;
             if (enter & (1u << q)) {
-                sd = stage1_seed(S, obj, o, d, r32, cull, c);
+                sd = stage1_seed(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
                 if (sd.have) { phase = kPhSeed; have_h = false; }   // parked until the wave runs Newton
             } else {
                 c.candidates++;
@@ -546,7 +546,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
 // are many) and uploads them as one device scene: patches, coarse data and
 // culling boxes concatenated in sub-scene order, each sub-scene's BVH appended
 // with its node, leaf and patch indices offset to global ones.
-int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, const char* who) {
+int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, const char* who, bool compact) {
     *out = nullptr;
     bbrt_scene s = nullptr;
     try {
@@ -556,7 +556,7 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
         const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::vector<bbrt::HostScene> H((size_t)n_sets);
         if (n_sets == 1) {
-            bbrt::build_host_scene(sets[0], H[0], (int)hw);
+            bbrt::build_host_scene(sets[0], H[0], (int)hw, compact);
         } else {
             // sub-scenes spread over the host threads, each built single-threaded
             std::atomic<int32_t> next{ 0 };
@@ -565,7 +565,7 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
             for (unsigned t = 0; t < hw; ++t)
                 pool.emplace_back([&]() {
                     for (int32_t i = next++; i < n_sets; i = next++) {
-                        try { bbrt::build_host_scene(sets[i], H[i], 1); }
+                        try { bbrt::build_host_scene(sets[i], H[i], 1, compact); }
                         catch (const std::exception& e) { errs[i] = e.what(); }
                     }
                 });
@@ -596,19 +596,34 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
         std::vector<int32_t> leaf;
         leaf.reserve((size_t)n_leaf);
         s->n_scenes = n_sets;
+        s->compact = compact;
         s->patch_begin.assign(1, 0);
         int64_t po = 0, no = 0, fo = 0;
+        int32_t so = 0;   // node-set offset of the sub-scene's tables
+        std::vector<double> ff_rows, mid_basis;
         for (int32_t i = 0; i < n_sets; ++i) {
             const auto& h = H[i];
             const int32_t bo = (int32_t)bvh.size(), lo = (int32_t)leaf.size();
-            for (PatchMeta m : h.meta) { m.node_off += no; m.field_off += fo; meta.push_back(m); }
+            if (compact && so + h.n_nodesets >= (1 << 16))
+                throw std::runtime_error(std::string(who) + ": more than 65535 distinct node sets");
+            for (PatchMeta m : h.meta) {
+                m.node_off += no; m.field_off += fo;
+                if (compact) m.nodeset = (((m.nodeset >> 16) + so) << 16) | ((m.nodeset & 0xFFFF) + so);
+                meta.push_back(m);
+            }
+            if (compact) {
+                ff_rows.insert(ff_rows.end(), h.ff_rows.begin(), h.ff_rows.end());
+                mid_basis.insert(mid_basis.end(), h.mid_basis.begin(), h.mid_basis.end());
+                so += h.n_nodesets;
+            }
             for (BvhNode nd : h.bvh) {
                 if (nd.right >= 0) nd.right += bo; else nd.start += lo;
                 bvh.push_back(nd);
             }
             for (int32_t o : h.leaf) leaf.push_back(o + (int32_t)po);
             bbrt::SceneInfo si{};
-            si.root = bo; si.patch_begin = (int32_t)po; si.diag = h.diag; si.cull_limit = 8.0 * h.scale; si.pad = 0.0;
+            si.root = bo; si.patch_begin = (int32_t)po; si.diag = h.diag; si.cull_limit = 8.0 * h.scale;
+            si.cull_widen = h.cull_widen;
             s->scenes.push_back(si);
             po += h.n; no += (int64_t)h.nodes.size(); fo += (int64_t)h.fields.size();
             s->patch_begin.push_back(po);
@@ -650,6 +665,9 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
         s->d_scenes = const_cast<bbrt::SceneInfo*>(dev_upload(s->scenes, s->allocs, s->bytes));
         D.scenes = s->d_scenes;
         D.ray_scene = nullptr;
+        D.ff_rows = compact ? dev_upload(ff_rows, s->allocs, s->bytes) : nullptr;
+        D.mid_basis = compact ? dev_upload(mid_basis, s->allocs, s->bytes) : nullptr;
+        D.compact = compact ? 1 : 0;
         s->n_patches = n_patch;
         for (auto& h : H) {
             s->pbox.insert(s->pbox.end(), h.pbox.begin(), h.pbox.end());
@@ -758,13 +776,20 @@ int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int
 
 int bbrt_scene_create(const bbrt_patch_set* ps, bbrt_scene* out) {
     if (!ps || !out) { g_last_error = "bbrt_scene_create: NULL argument"; return -1; }
-    return create_scenes(ps, 1, out, "bbrt_scene_create");
+    return create_scenes(ps, 1, out, "bbrt_scene_create", false);
 }
 
 int bbrt_scene_create_batch(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out) {
     if (!sets || !out) { g_last_error = "bbrt_scene_create_batch: NULL argument"; return -1; }
     if (n_sets <= 0) { g_last_error = "bbrt_scene_create_batch: no sub-scenes"; return -1; }
-    return create_scenes(sets, n_sets, out, "bbrt_scene_create_batch");
+    return create_scenes(sets, n_sets, out, "bbrt_scene_create_batch", false);
+}
+
+int bbrt_scene_create_ex(const bbrt_patch_set* sets, int32_t n_sets, uint32_t scene_flags, bbrt_scene* out) {
+    if (!sets || !out) { g_last_error = "bbrt_scene_create_ex: NULL argument"; return -1; }
+    if (n_sets <= 0) { g_last_error = "bbrt_scene_create_ex: no sub-scenes"; return -1; }
+    if (scene_flags & ~(uint32_t)BBRT_SCENE_COMPACT) { g_last_error = "bbrt_scene_create_ex: unknown flag"; return -1; }
+    return create_scenes(sets, n_sets, out, "bbrt_scene_create_ex", (scene_flags & BBRT_SCENE_COMPACT) != 0);
 }
 
 int bbrt_scene_batch_info(bbrt_scene s, int32_t* n_scenes, int64_t* patch_begin, int32_t* bvh_root, double* diag) {
@@ -817,6 +842,7 @@ int bbrt_scene_bvh(bbrt_scene s, double* node_box, int32_t* right, int32_t* star
 int bbrt_scene_coarse(bbrt_scene s, int64_t k, double* xyz, double* sag, double* cnorm) {
     if (!s) { g_last_error = "bbrt_scene_coarse: NULL scene"; return -1; }
     if (k < 0 || k >= s->n_patches) { g_last_error = "bbrt_scene_coarse: patch index out of range"; return -1; }
+    if (s->compact) { g_last_error = "bbrt_scene_coarse: a compact scene keeps no coarse grid"; return -1; }
     try {
         check(hipSetDevice(s->device), "hipSetDevice");
         const size_t nv = (size_t)bbrt::kCoarseN * bbrt::kCoarseN * 3;

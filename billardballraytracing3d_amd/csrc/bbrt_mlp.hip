@@ -192,7 +192,7 @@ __global__ void k_mlp_gen(DevScene S, const double* frame, const float* pfeat, i
     if (g >= n) return;
     const Pair pr = draw_pair(S, frame, n_patch, seed, first + g);
     Counters c = {};
-    const Seed sd = stage1_seed(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, c);
+    const Seed sd = stage1_seed(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, S.scenes[0].cull_widen, c);
     PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     if (sd.have) h = newton_any(S, pr.patch, pr.o, pr.d, sd.u, sd.v, sd.t, BBRT_NEWTON_CONVERGED, c);
     const bool ok = h.status == 1;
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256) k_mlp_ab(DevScene S, MlpDev M, int64_t n_
     PHit ht{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     if (in) {
         pr = draw_pair(S, M.frame, n_patch, seed, first + g);
-        sd = stage1_seed(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, c);
+        sd = stage1_seed(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, S.scenes[0].cull_widen, c);
         if (sd.have) ht = newton_any(S, pr.patch, pr.o, pr.d, sd.u, sd.v, sd.t, BBRT_NEWTON_CONVERGED, c);
     }
     double ul, vl, tl;
@@ -819,6 +819,7 @@ int bbrt_mlp_train(bbrt_mlp m, bbrt_scene s, const bbrt_train_config* cfg, float
         g_last_error = "bbrt_mlp_train: invalid configuration";
         return -1;
     }
+    if (s->n_scenes != 1) { g_last_error = "bbrt_mlp_train: draws pairs from a single scene"; return -1; }
     try {
         check(hipSetDevice(s->device), "hipSetDevice");
         if (m->device != s->device) throw std::runtime_error("bbrt_mlp_train: network and scene on different devices");
@@ -907,6 +908,7 @@ int bbrt_mlp_sample_pairs(bbrt_scene s, uint64_t seed, int64_t first, int64_t n,
         return -1;
     }
     if (n < 0 || n > (int64_t)1 << 24 || first < 0) { g_last_error = "bbrt_mlp_sample_pairs: n out of range"; return -1; }
+    if (s->n_scenes != 1) { g_last_error = "bbrt_mlp_sample_pairs: draws pairs from a single scene"; return -1; }
     if (n == 0) return 0;
     try {
         ensure_frames(s);
@@ -1006,6 +1008,7 @@ int bbrt_mlp_seed_ab(bbrt_scene s, uint64_t seed, int64_t first, int64_t n, int8
     }
     if (!s->mlp_ready) { g_last_error = "bbrt_mlp_seed_ab: no network attached (bbrt_scene_set_mlp)"; return -1; }
     if (n < 0 || n > (int64_t)1 << 26 || first < 0) { g_last_error = "bbrt_mlp_seed_ab: n out of range"; return -1; }
+    if (s->n_scenes != 1) { g_last_error = "bbrt_mlp_seed_ab: draws pairs from a single scene"; return -1; }
     if (n == 0) return 0;
     try {
         check(hipSetDevice(s->device), "hipSetDevice");

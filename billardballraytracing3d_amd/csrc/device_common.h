@@ -43,6 +43,10 @@ struct DevScene {
     const int32_t* leaf;
     const bbrt::SceneInfo* scenes;   // per sub-scene: BVH root, first patch, diagonal, culling limit
     const int32_t* ray_scene;        // per ray: its sub-scene (NULL: every ray in sub-scene 0)
+    // compact scenes: no cxyz / cell / box_cell; cells are evaluated from the node-set tables
+    const double* ff_rows;           // [set][kCoarseN][kRowStride]
+    const double* mid_basis;         // [set][kCells][kMaxNodes1D]
+    int compact;
 };
 
 __device__ __forceinline__ int ray_scene_of(const DevScene& S, int64_t r) { return S.ray_scene ? S.ray_scene[r] : 0; }
@@ -179,6 +183,136 @@ __device__ __forceinline__ void cell_tests(const double* __restrict__ G, const d
     c.mt_tests += 6;
 }
 
+// ---------------------------------------------------------------------------
+// Compact scenes: a coarse cell evaluated from the nodal fields, bit-identical
+// to the host build (scene_build.cpp compute_grid / compute_cells / build_pyramid).
+// ---------------------------------------------------------------------------
+struct CellData {
+    V3 p00, p10, p01, p11;   // corners (i,j), (i+1,j), (i,j+1), (i+1,j+1)
+    double s;                // sagitta
+    V3 n;                    // unit normal at the cell midpoint
+};
+
+// Coarse vertex (i, j): (sum_m (sum_q ru[i][q] f[q][m]) rv[j][m]) * (su[i] sv[j]) per coordinate,
+// first-form interpolation (PatchInterpolation.hpp:54-159) with the inner index summed upwards.
+__device__ __forceinline__ V3 coarse_point(const DevScene& S, const PatchMeta& M, int i, int j) {
+    const int im = M.imax, jm = M.jmax, nn = im * jm;
+    const double* ru = S.ff_rows + ((size_t)(M.nodeset >> 16) * bbrt::kCoarseN + i) * bbrt::kRowStride;
+    const double* rv = S.ff_rows + ((size_t)(M.nodeset & 0xFFFF) * bbrt::kCoarseN + j) * bbrt::kRowStride;
+    const double* F = S.fields + M.field_off;
+    const double sc = ru[bbrt::kMaxNodes1D] * rv[bbrt::kMaxNodes1D];
+    double out[3];
+    for (int c = 0; c < 3; ++c) {
+        const double* f = F + c * nn;
+        double acc2 = 0.0;
+        for (int m = 0; m < jm; ++m) {
+            double acc = 0.0;
+            for (int q = 0; q < im; ++q) acc += ru[q] * f[q * jm + m];
+            acc2 += acc * rv[m];
+        }
+        out[c] = acc2 * sc;
+    }
+    return { out[0], out[1], out[2] };
+}
+
+__device__ __forceinline__ double field_tab(const double* lu, const double* lv, const double* f, int im, int jm) {
+    double sum = 0.0;
+    for (int i = 0; i < im; ++i) {
+        const double li = lu[i];
+        if (li != 0.0) {
+            double row = 0.0;
+            for (int j = 0; j < jm; ++j) row += lv[j] * f[i * jm + j];
+            sum += li * row;
+        }
+    }
+    return sum;
+}
+
+// Corners, sagitta and normal of cell (i, j) (SurfacePatch.hpp:416-492).
+__device__ CellData compact_cell(const DevScene& S, int64_t k, int i, int j) {
+    const PatchMeta M = S.meta[k];
+    CellData cd;
+    cd.p00 = coarse_point(S, M, i, j);
+    cd.p10 = coarse_point(S, M, i + 1, j);
+    cd.p01 = coarse_point(S, M, i, j + 1);
+    cd.p11 = coarse_point(S, M, i + 1, j + 1);
+    const int im = M.imax, jm = M.jmax, nn = im * jm;
+    const double* lu = S.mid_basis + ((size_t)(M.nodeset >> 16) * bbrt::kCells + i) * bbrt::kMaxNodes1D;
+    const double* lv = S.mid_basis + ((size_t)(M.nodeset & 0xFFFF) * bbrt::kCells + j) * bbrt::kMaxNodes1D;
+    const double* F = S.fields + M.field_off;
+    const V3 tm = { field_tab(lu, lv, F, im, jm), field_tab(lu, lv, F + nn, im, jm), field_tab(lu, lv, F + 2 * nn, im, jm) };
+    cd.n = unit3({ field_tab(lu, lv, F + 9 * nn, im, jm), field_tab(lu, lv, F + 10 * nn, im, jm),
+                   field_tab(lu, lv, F + 11 * nn, im, jm) });
+    const V3 bm = { 0.25 * (cd.p00.x + cd.p10.x + cd.p01.x + cd.p11.x), 0.25 * (cd.p00.y + cd.p10.y + cd.p01.y + cd.p11.y),
+                    0.25 * (cd.p00.z + cd.p10.z + cd.p01.z + cd.p11.z) };
+    const V3 dd = sub3(tm, bm);
+    cd.s = sqrt(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z);
+    return cd;
+}
+
+__device__ __forceinline__ float round_down_f(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ __forceinline__ float round_up_f(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+// The cell's FP32 culling box exactly as build_pyramid makes it, and the slab test.
+__device__ __forceinline__ bool compact_cell_box(const CellData& cd, double widen, const Ray32& r) {
+    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    const double signs[3] = { 0.0, 1.0, -1.0 };
+    const V3* corner[4] = { &cd.p00, &cd.p10, &cd.p01, &cd.p11 };
+    for (int si = 0; si < 3; ++si) {
+        const double dd = signs[si] * cd.s;
+        for (int q = 0; q < 4; ++q) {
+            const double x = corner[q]->x + cd.n.x * dd, y = corner[q]->y + cd.n.y * dd, z = corner[q]->z + cd.n.z * dd;
+            lo[0] = (x < lo[0]) ? x : lo[0]; hi[0] = (hi[0] < x) ? x : hi[0];
+            lo[1] = (y < lo[1]) ? y : lo[1]; hi[1] = (hi[1] < y) ? y : hi[1];
+            lo[2] = (z < lo[2]) ? z : lo[2]; hi[2] = (hi[2] < z) ? z : hi[2];
+        }
+    }
+    const float b[6] = { round_down_f(lo[0] - widen), round_down_f(lo[1] - widen), round_down_f(lo[2] - widen),
+                         round_up_f(hi[0] + widen), round_up_f(hi[1] + widen), round_up_f(hi[2] + widen) };
+    return box32(b, r);
+}
+
+// The six Moller-Trumbore tests of a cell given its data (SurfacePatch.hpp:274-294).
+__device__ __forceinline__ void cell_tests_data(const CellData& cd, int i, int j, const V3& o, const V3& d,
+                                                Stage1& best, Counters& c) {
+    const int base = (i * bbrt::kCells + j) * 6;
+    const double signs[3] = { 0.0, 1.0, -1.0 };
+#pragma unroll
+    for (int si = 0; si < 3; ++si) {
+        const double dd = signs[si] * cd.s;
+        const V3 q00 = { cd.p00.x + cd.n.x * dd, cd.p00.y + cd.n.y * dd, cd.p00.z + cd.n.z * dd };
+        const V3 q10 = { cd.p10.x + cd.n.x * dd, cd.p10.y + cd.n.y * dd, cd.p10.z + cd.n.z * dd };
+        const V3 q01 = { cd.p01.x + cd.n.x * dd, cd.p01.y + cd.n.y * dd, cd.p01.z + cd.n.z * dd };
+        const V3 q11 = { cd.p11.x + cd.n.x * dd, cd.p11.y + cd.n.y * dd, cd.p11.z + cd.n.z * dd };
+        double t;
+        int key = base + 2 * si;
+        if (mt_t(o, d, q00, q10, q11, t) && (!best.have || t < best.t || (t == best.t && key < best.key))) {
+            best.have = true; best.t = t; best.key = key;
+        }
+        key += 1;
+        if (mt_t(o, d, q00, q11, q01, t) && (!best.have || t < best.t || (t == best.t && key < best.key))) {
+            best.have = true; best.t = t; best.key = key;
+        }
+    }
+    c.mt_tests += 6;
+}
+
+__device__ void stage1_brute_compact(const DevScene& S, int64_t k, const V3& o, const V3& d, Stage1& best, Counters& c) {
+    for (int i = 0; i < bbrt::kCells; ++i)
+        for (int j = 0; j < bbrt::kCells; ++j) {
+            const CellData cd = compact_cell(S, k, i, j);
+            cell_tests_data(cd, i, j, o, d, best, c);
+        }
+}
+
 // Stage 1 over the whole coarse grid in the reference's order (brute force).
 __device__ void stage1_brute(const double* G, const double* C, const V3& o, const V3& d, Stage1& best, Counters& c) {
     for (int i = 0; i < bbrt::kCells; ++i)
@@ -205,7 +339,7 @@ __device__ __forceinline__ uint32_t group_mask(const float* __restrict__ g, cons
 
 // Stage 1 through the culling pyramid (grouped layout: one load per level).
 __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, const double* C, const V3& o,
-                              const V3& d, const Ray32& r, Stage1& best, Counters& c) {
+                              const V3& d, const Ray32& r, double widen, Stage1& best, Counters& c) {
     const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
     const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
     const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
@@ -228,6 +362,16 @@ __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, con
                 const int tq = __ffs(mt) - 1;
                 mt &= mt - 1;
                 const int ti = 3 * bi + tq / 3, tj = 3 * bj + tq % 3;
+                if (S.compact) {
+                    // the tile's nine cells evaluated from the fields; each box rebuilt and tested as stored
+                    for (int cq = 0; cq < 9; ++cq) {
+                        const int i = 3 * ti + cq / 3, j = 3 * tj + cq % 3;
+                        const CellData cd = compact_cell(S, k, i, j);
+                        if (compact_cell_box(cd, widen, r)) cell_tests_data(cd, i, j, o, d, best, c);
+                    }
+                    c.cull_tests += 9;
+                    continue;
+                }
                 uint32_t mc = group_mask<9, bbrt::kG9>(gc + (ti * bbrt::kTileN + tj) * bbrt::kG9, r);
                 c.cull_tests += 9;
                 while (mc) {
@@ -425,12 +569,13 @@ struct Seed {
 };
 
 __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32,
-                                            bool cull, Counters& c) {
+                                            bool cull, double widen, Counters& c) {
     const double* G = S.cxyz + (size_t)k * bbrt::kCoarseN * bbrt::kCoarseN * 3;
     const double* C = S.cell + (size_t)k * bbrt::kCells * bbrt::kCells * 4;
     Stage1 best{ 0.0, 0, false };
     c.candidates++;
-    if (cull && S.meta[k].cull) stage1_culled(S, k, G, C, o, d, r32, best, c);
+    if (cull && S.meta[k].cull) stage1_culled(S, k, G, C, o, d, r32, widen, best, c);
+    else if (S.compact) stage1_brute_compact(S, k, o, d, best, c);
     else stage1_brute(G, C, o, d, best, c);
     Seed sd{ 0.0, 0.0, best.t, best.have };
     if (!best.have) return sd;
@@ -457,8 +602,8 @@ __device__ __forceinline__ PHit newton_any(const DevScene& S, int64_t k, const V
 
 // SurfacePatch::intersect (SurfacePatch.hpp:253-303): stage 1, then Newton from its seed.
 __device__ PHit patch_intersect(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32, bool cull,
-                                int mode, Counters& c) {
-    const Seed sd = stage1_seed(S, k, o, d, r32, cull, c);
+                                double widen, int mode, Counters& c) {
+    const Seed sd = stage1_seed(S, k, o, d, r32, cull, widen, c);
     if (!sd.have) return PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     return newton_any(S, k, o, d, sd.u, sd.v, sd.t, mode, c);
 }
@@ -544,8 +689,9 @@ __device__ __forceinline__ Ent make_ent(int32_t idx, const BvhNode& nd, bool ok,
 // dead entry) so the visit order, the stack depth and the node-test count are
 // the reference's.  The near child stays in registers instead of being pushed
 // and popped at once.  Returns 0 or an error kind.
-__device__ int traverse(const DevScene& S, int32_t root_idx, const V3& o, const V3& d, const Ray32& r32, bool cull,
-                        int mode, bool root_live, PHit& best, int32_t& best_obj, int32_t& err_obj, Counters& c) {
+__device__ int traverse(const DevScene& S, int32_t root_idx, double widen, const V3& o, const V3& d, const Ray32& r32,
+                        bool cull, int mode, bool root_live, PHit& best, int32_t& best_obj, int32_t& err_obj,
+                        Counters& c) {
     const double inv[3] = { 1.0 / d.x, 1.0 / d.y, 1.0 / d.z };
     const bool neg[3] = { inv[0] < 0.0, inv[1] < 0.0, inv[2] < 0.0 };
     double best_t = INFINITY;
@@ -613,7 +759,7 @@ __device__ int traverse(const DevScene& S, int32_t root_idx, const V3& o, const 
             for (int q = 0; q < 4 && q0 + q < count; ++q) {
                 const int32_t obj = ids[q];
                 if (!(enter & (1u << q))) { c.candidates++; continue; }
-                const PHit h = patch_intersect(S, obj, o, d, r32, cull, mode, c);
+                const PHit h = patch_intersect(S, obj, o, d, r32, cull, widen, mode, c);
                 if (h.status < 0) { err_obj = obj; return -h.status; }
                 if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
             }
@@ -704,7 +850,8 @@ __device__ bool bounce_once(const DevScene& S, const Outs& O, int32_t r, V3& o, 
     const Ray32 r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
     PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     int32_t best_obj, err_obj;
-    const int err = traverse(S, S.scenes[scn].root, o, d, r32, cull, mode, root_live, best, best_obj, err_obj, c);
+    const int err = traverse(S, S.scenes[scn].root, S.scenes[scn].cull_widen, o, d, r32, cull, mode, root_live, best,
+                             best_obj, err_obj, c);
     const bool alive = finish_bounce(S, O, r, scn, o, d, bounce, max_bounces, record_k, flags, err, err_obj, best,
                                      best_obj, c);
     if constexpr (WORK) add_ray_work(O, r, c, c0);

@@ -82,6 +82,7 @@ struct bbrt_scene_s {
     int64_t graph_captures = 0;
     // sub-scenes (one for bbrt_scene_create, many for bbrt_scene_create_batch)
     int32_t n_scenes = 1;
+    bool compact = false;
     std::vector<bbrt::SceneInfo> scenes;
     bbrt::SceneInfo* d_scenes = nullptr;
     std::vector<int64_t> patch_begin;   // [n_scenes + 1]

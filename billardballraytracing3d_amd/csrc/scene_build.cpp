@@ -24,6 +24,7 @@
 #include <atomic>
 #include <cmath>
 #include <limits>
+#include <map>
 #include <stdexcept>
 #include <thread>
 
@@ -113,6 +114,123 @@ struct PatchWork {
     std::vector<double> hmax;   // per patch: longest coarse cell diagonal
 };
 
+// Per-thread coarse grid and cell padding of one patch, for compact scenes
+// (which keep neither).
+double* tls_grid() {
+    thread_local std::vector<double> g;
+    if (g.empty()) g.resize((size_t)kCoarseN * kCoarseN * 3);
+    return g.data();
+}
+double* tls_cells() {
+    thread_local std::vector<double> c;
+    if (c.empty()) c.resize((size_t)kCells * kCells * 4);
+    return c.data();
+}
+
+// Coarse grid of patch k into G (SurfacePatch.hpp:416-436).
+void compute_grid(const HostScene& S, int64_t k, double* G) {
+    const PatchMeta& M = S.meta[k];
+    const int im = M.imax, jm = M.jmax;
+    const double* un = &S.nodes[M.node_off];
+    const double* uw = un + im;
+    const double* vn = uw + im;
+    const double* vw = vn + jm;
+    const double* F = &S.fields[M.field_off];
+    const size_t nn = (size_t)im * jm;
+    double cu[kCoarseN];
+    for (int i = 0; i < kCoarseN; ++i) cu[i] = -1.0 + 2.0 * static_cast<double>(i) / (kCoarseN - 1);
+    std::vector<double> ru, su, rv, sv;
+    first_form_rows(un, uw, im, cu, kCoarseN, ru, su);
+    first_form_rows(vn, vw, jm, cu, kCoarseN, rv, sv);
+    std::vector<double> mid((size_t)kCoarseN * jm);
+    for (int c = 0; c < 3; ++c) {
+        const double* f = F + c * nn;
+        for (int i = 0; i < kCoarseN; ++i)
+            for (int m = 0; m < jm; ++m) {
+                double acc = 0.0;
+                for (int q = 0; q < im; ++q) acc += ru[(size_t)i * im + q] * f[(size_t)q * jm + m];
+                mid[(size_t)i * jm + m] = acc;
+            }
+        for (int i = 0; i < kCoarseN; ++i)
+            for (int j = 0; j < kCoarseN; ++j) {
+                double acc = 0.0;
+                for (int m = 0; m < jm; ++m) acc += mid[(size_t)i * jm + m] * rv[(size_t)j * jm + m];
+                G[((size_t)i * kCoarseN + j) * 3 + c] = acc * (su[i] * sv[j]);
+            }
+    }
+}
+
+// Per-cell sagitta and unit normal of patch k into C (SurfacePatch.hpp:446-492).
+void compute_cells(const HostScene& S, int64_t k, const double* G, double* C) {
+    const PatchMeta& M = S.meta[k];
+    const int im = M.imax, jm = M.jmax;
+    const double* un = &S.nodes[M.node_off];
+    const double* uw = un + im;
+    const double* vn = uw + im;
+    const double* vw = vn + jm;
+    const double* F = &S.fields[M.field_off];
+    const size_t nn = (size_t)im * jm;
+    double cu[kCoarseN];
+    for (int i = 0; i < kCoarseN; ++i) cu[i] = -1.0 + 2.0 * static_cast<double>(i) / (kCoarseN - 1);
+    double lu[kMaxNodes1D], lv[kMaxNodes1D];
+    for (int i = 0; i < kCells; ++i)
+        for (int j = 0; j < kCells; ++j) {
+            const double um = 0.5 * (cu[i] + cu[i + 1]);
+            const double vm = 0.5 * (cu[j] + cu[j + 1]);
+            basis_at(un, uw, im, um, lu);
+            basis_at(vn, vw, jm, vm, lv);
+            const P3 tm = { field_at(lu, lv, F, im, jm), field_at(lu, lv, F + nn, im, jm),
+                            field_at(lu, lv, F + 2 * nn, im, jm) };
+            const P3 nm = unit({ field_at(lu, lv, F + 9 * nn, im, jm), field_at(lu, lv, F + 10 * nn, im, jm),
+                                 field_at(lu, lv, F + 11 * nn, im, jm) });
+            const double* p00 = &G[((size_t)i * kCoarseN + j) * 3];
+            const double* p10 = &G[((size_t)(i + 1) * kCoarseN + j) * 3];
+            const double* p01 = &G[((size_t)i * kCoarseN + j + 1) * 3];
+            const double* p11 = &G[((size_t)(i + 1) * kCoarseN + j + 1) * 3];
+            const P3 bm = { 0.25 * (p00[0] + p10[0] + p01[0] + p11[0]), 0.25 * (p00[1] + p10[1] + p01[1] + p11[1]),
+                            0.25 * (p00[2] + p10[2] + p01[2] + p11[2]) };
+            const P3 d = { tm.x - bm.x, tm.y - bm.y, tm.z - bm.z };
+            const double s = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            double* c = &C[((size_t)i * kCells + j) * 4];
+            c[0] = s; c[1] = nm.x; c[2] = nm.y; c[3] = nm.z;
+        }
+}
+
+// Node-set tables of a compact scene: for each distinct node vector, the
+// first-form rows (values then scale) at the 100 coarse values and the
+// second-form basis at the 99 cell midpoints -- exactly what compute_grid and
+// compute_cells evaluate per patch, shared by all patches with those nodes.
+struct NodeSetTable {
+    HostScene& S;
+    std::map<std::vector<double>, int> index;
+    int get(const double* x, const double* w, int n) {
+        std::vector<double> key(x, x + n);
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        const int id = (int)index.size();
+        if (id >= (1 << 15)) throw std::runtime_error("compact scene: more than 32768 distinct node sets");
+        index.emplace(std::move(key), id);
+        double cu[kCoarseN];
+        for (int i = 0; i < kCoarseN; ++i) cu[i] = -1.0 + 2.0 * static_cast<double>(i) / (kCoarseN - 1);
+        std::vector<double> rows, scale;
+        first_form_rows(x, w, n, cu, kCoarseN, rows, scale);
+        const size_t r0 = S.ff_rows.size();
+        S.ff_rows.resize(r0 + (size_t)kCoarseN * kRowStride, 0.0);
+        for (int i = 0; i < kCoarseN; ++i) {
+            for (int q = 0; q < n; ++q) S.ff_rows[r0 + (size_t)i * kRowStride + q] = rows[(size_t)i * n + q];
+            S.ff_rows[r0 + (size_t)i * kRowStride + kMaxNodes1D] = scale[i];
+        }
+        const size_t m0 = S.mid_basis.size();
+        S.mid_basis.resize(m0 + (size_t)kCells * kMaxNodes1D, 0.0);
+        for (int i = 0; i < kCells; ++i) {
+            const double um = 0.5 * (cu[i] + cu[i + 1]);
+            basis_at(x, w, n, um, &S.mid_basis[m0 + (size_t)i * kMaxNodes1D]);
+        }
+        S.n_nodesets = (int)index.size();
+        return id;
+    }
+};
+
 void prepare_patch(PatchWork& W, int64_t k) {
     const bbrt_patch_set& ps = *W.ps;
     HostScene& S = *W.S;
@@ -145,55 +263,20 @@ void prepare_patch(PatchWork& W, int64_t k) {
     // coarse grid (SurfacePatch.hpp:416-436)
     double cu[kCoarseN];
     for (int i = 0; i < kCoarseN; ++i) cu[i] = -1.0 + 2.0 * static_cast<double>(i) / (kCoarseN - 1);
-    std::vector<double> ru, su, rv, sv;
-    first_form_rows(un, uw, im, cu, kCoarseN, ru, su);
-    first_form_rows(vn, vw, jm, cu, kCoarseN, rv, sv);
-    double* G = &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
-    std::vector<double> mid((size_t)kCoarseN * jm);
-    for (int c = 0; c < 3; ++c) {
-        const double* f = F + c * nn;
-        for (int i = 0; i < kCoarseN; ++i)
-            for (int m = 0; m < jm; ++m) {
-                double acc = 0.0;
-                for (int q = 0; q < im; ++q) acc += ru[(size_t)i * im + q] * f[(size_t)q * jm + m];
-                mid[(size_t)i * jm + m] = acc;
-            }
-        for (int i = 0; i < kCoarseN; ++i)
-            for (int j = 0; j < kCoarseN; ++j) {
-                double acc = 0.0;
-                for (int m = 0; m < jm; ++m) acc += mid[(size_t)i * jm + m] * rv[(size_t)j * jm + m];
-                G[((size_t)i * kCoarseN + j) * 3 + c] = acc * (su[i] * sv[j]);
-            }
-    }
+    double* G = S.compact ? tls_grid() : &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
+    compute_grid(S, k, G);
 
     // cell padding (SurfacePatch.hpp:446-492)
-    double* C = &S.cell[(size_t)k * kCells * kCells * 4];
-    double lu[kMaxNodes1D], lv[kMaxNodes1D];
+    double* C = S.compact ? tls_cells() : &S.cell[(size_t)k * kCells * kCells * 4];
+    compute_cells(S, k, G, C);
+    (void)un; (void)uw; (void)vn; (void)vw; (void)F; (void)nn; (void)cu;
     double ms = 0.0;
     bool finite = true;
-    for (int i = 0; i < kCells; ++i)
-        for (int j = 0; j < kCells; ++j) {
-            const double um = 0.5 * (cu[i] + cu[i + 1]);
-            const double vm = 0.5 * (cu[j] + cu[j + 1]);
-            basis_at(un, uw, im, um, lu);
-            basis_at(vn, vw, jm, vm, lv);
-            const P3 tm = { field_at(lu, lv, F, im, jm), field_at(lu, lv, F + nn, im, jm),
-                            field_at(lu, lv, F + 2 * nn, im, jm) };
-            const P3 nm = unit({ field_at(lu, lv, F + 9 * nn, im, jm), field_at(lu, lv, F + 10 * nn, im, jm),
-                                 field_at(lu, lv, F + 11 * nn, im, jm) });
-            const double* p00 = &G[((size_t)i * kCoarseN + j) * 3];
-            const double* p10 = &G[((size_t)(i + 1) * kCoarseN + j) * 3];
-            const double* p01 = &G[((size_t)i * kCoarseN + j + 1) * 3];
-            const double* p11 = &G[((size_t)(i + 1) * kCoarseN + j + 1) * 3];
-            const P3 bm = { 0.25 * (p00[0] + p10[0] + p01[0] + p11[0]), 0.25 * (p00[1] + p10[1] + p01[1] + p11[1]),
-                            0.25 * (p00[2] + p10[2] + p01[2] + p11[2]) };
-            const P3 d = { tm.x - bm.x, tm.y - bm.y, tm.z - bm.z };
-            const double s = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-            double* c = &C[((size_t)i * kCells + j) * 4];
-            c[0] = s; c[1] = nm.x; c[2] = nm.y; c[3] = nm.z;
-            ms = greater(ms, s);
-            if (!(std::isfinite(s) && std::isfinite(nm.x) && std::isfinite(nm.y) && std::isfinite(nm.z))) finite = false;
-        }
+    for (size_t q = 0; q < (size_t)kCells * kCells; ++q) {
+        const double* c = &C[q * 4];
+        ms = greater(ms, c[0]);
+        if (!(std::isfinite(c[0]) && std::isfinite(c[1]) && std::isfinite(c[2]) && std::isfinite(c[3]))) finite = false;
+    }
     for (size_t q = 0; q < (size_t)kCoarseN * kCoarseN * 3; ++q)
         if (!std::isfinite(G[q])) finite = false;
     double* b = &S.pbox[6 * k];
@@ -221,9 +304,24 @@ void prepare_patch(PatchWork& W, int64_t k) {
 // FP32 culling pyramid of one patch, widened by `widen`, in the grouped layout
 // described in scene_host.h.
 void build_pyramid(HostScene& S, int64_t k, double widen) {
-    const double* G = &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
-    const double* C = &S.cell[(size_t)k * kCells * kCells * 4];
-    float* gc = &S.box_cell[(size_t)k * kCellFloats];
+    const double* G;
+    const double* C;
+    float* gc;
+    std::vector<float> gc_local;
+    if (S.compact) {   // recompute the grid and the cell padding; the cell boxes are not kept
+        double* g = tls_grid();
+        double* cp = tls_cells();
+        compute_grid(S, k, g);
+        compute_cells(S, k, g, cp);
+        G = g;
+        C = cp;
+        gc_local.assign(kCellFloats, 0.0f);
+        gc = gc_local.data();
+    } else {
+        G = &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
+        C = &S.cell[(size_t)k * kCells * kCells * 4];
+        gc = &S.box_cell[(size_t)k * kCellFloats];
+    }
     float* gt = &S.box_tile[(size_t)k * kTileFloats];
     float* gb = &S.box_block[(size_t)k * kBlockFloats];
     float* gs = &S.box_super[(size_t)k * kSuperFloats];
@@ -378,7 +476,7 @@ void parallel_for(int64_t n, int threads, F&& fn) {
 
 }  // namespace
 
-void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads) {
+void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads, bool compact) {
     const int64_t n = ps.n_patches;
     if (n <= 0) throw std::runtime_error("bbrt_scene_create: no patches");
     if (!ps.imax || !ps.jmax || !ps.u_off || !ps.v_off || !ps.f_off || !ps.unodes || !ps.vnodes || !ps.fields)
@@ -397,7 +495,7 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads) {
         M.imax = im; M.jmax = jm;
         M.node_off = node_total;
         M.field_off = 12 * ps.f_off[k];
-        M.max_sag = 0.0; M.cull = 0; M.pad0 = 0; M.pad1 = 0.0;
+        M.max_sag = 0.0; M.cull = 0; M.nodeset = 0; M.pad1 = 0.0;
         node_total += 2 * (im + jm);
     }
     S.nodes.resize(node_total);
@@ -421,8 +519,20 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads) {
         for (int j = 0; j < M.jmax; ++j) dst[2 * M.imax + j] = v[j];
         weights(v, M.jmax, dst + 2 * M.imax + M.jmax);
     }
-    S.cxyz.assign((size_t)n * kCoarseN * kCoarseN * 3, 0.0);
-    S.cell.assign((size_t)n * kCells * kCells * 4, 0.0);
+    S.compact = compact;
+    if (compact) {
+        NodeSetTable T{ S, {} };
+        for (int64_t k = 0; k < n; ++k) {
+            PatchMeta& M = S.meta[k];
+            const double* un = &S.nodes[M.node_off];
+            const double* vn = un + 2 * M.imax;
+            const int us = T.get(un, un + M.imax, M.imax), vs = T.get(vn, vn + M.jmax, M.jmax);
+            M.nodeset = (us << 16) | vs;
+        }
+    } else {
+        S.cxyz.assign((size_t)n * kCoarseN * kCoarseN * 3, 0.0);
+        S.cell.assign((size_t)n * kCells * kCells * 4, 0.0);
+    }
     S.pbox.assign((size_t)n * 6, 0.0);
     S.pcent.assign((size_t)n * 3, 0.0);
     PatchWork W{ &ps, &S, std::vector<double>(n, 0.0) };
@@ -454,7 +564,7 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads) {
     // max(1, h^2) exceeds both by more than an order of magnitude.
     S.cull_widen = 1e-5 * scale * std::max(1.0, hmax * hmax);
 
-    S.box_cell.assign((size_t)n * kCellFloats, 0.0f);
+    if (!compact) S.box_cell.assign((size_t)n * kCellFloats, 0.0f);
     S.box_tile.assign((size_t)n * kTileFloats, 0.0f);
     S.box_block.assign((size_t)n * kBlockFloats, 0.0f);
     S.box_super.assign((size_t)n * kSuperFloats, 0.0f);

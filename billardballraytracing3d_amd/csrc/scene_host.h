@@ -41,6 +41,8 @@ enum : int {
     kCtrNewtonFlops = BBRT_CTR_NEWTON_FLOPS, kCtrPolishFail = BBRT_CTR_POLISH_FAIL, kCtrLaunches = BBRT_CTR_LAUNCHES
 };
 
+constexpr int kRowStride = kMaxNodes1D + 1;           // node-set table row: first-form row values, then scale
+
 // Per-patch record, 48 bytes.
 struct PatchMeta {
     int32_t imax, jmax;
@@ -48,7 +50,7 @@ struct PatchMeta {
     int64_t field_off;   // into fields[]: 12 * f_off[k]
     double max_sag;
     int32_t cull;        // 1 when every coarse value is finite (culling pyramid usable)
-    int32_t pad0;
+    int32_t nodeset;     // compact scenes: (u node set << 16) | v node set, into the node-set tables
     double pad1;
 };
 
@@ -58,7 +60,7 @@ struct SceneInfo {
     int32_t patch_begin;   // first global patch index
     double diag;           // scene diagonal (RayTracingDriver.hpp:386-396)
     double cull_limit;     // culling is used while |origin|_inf <= cull_limit
-    double pad;
+    double cull_widen;     // absolute widening of the FP32 culling boxes (compact scenes rebuild cell boxes)
 };
 
 // BVH node, 64 bytes, the reference's depth-first layout (BVH.hpp:56-62).
@@ -86,10 +88,17 @@ struct HostScene {
     double diag = 0.0, max_patch_sag = 0.0;
     double scale = 1.0;             // max(1, |coordinate|) over all patch boxes
     double cull_widen = 0.0;        // absolute widening of the FP32 culling boxes
+    // Compact scenes keep neither the coarse grid, the cell padding nor the cell
+    // boxes (cxyz, cell, box_cell stay empty): the kernels evaluate a cell from
+    // the nodal fields and these per-node-set tables.
+    bool compact = false;
+    std::vector<double> ff_rows;    // [set][kCoarseN][kRowStride]: first-form rows at the coarse values, scale
+    std::vector<double> mid_basis;  // [set][kCells][kMaxNodes1D]: second-form basis at the cell midpoints
+    int n_nodesets = 0;
 };
 
 // Builds everything above from a patch set; throws std::runtime_error on bad input.
-void build_host_scene(const bbrt_patch_set& ps, HostScene& out, int threads);
+void build_host_scene(const bbrt_patch_set& ps, HostScene& out, int threads, bool compact = false);
 
 // BVH::build (BVH.hpp:136-189) over objects with boxes [n][6] and centroids [n][3].
 void build_bvh(const std::vector<double>& box, const std::vector<double>& cent, int leaf_threshold,

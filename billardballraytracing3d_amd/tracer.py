@@ -83,13 +83,14 @@ class GpuScene:
         self.patch_begin = np.array([0, sc.n_patches], np.int64)
 
     @classmethod
-    def batch(cls, scenes):
-        """Several independent scenes in one handle (bbrt_scene_create_batch)."""
+    def batch(cls, scenes, compact: bool = False):
+        """Several independent scenes in one handle (bbrt_scene_create_ex; compact: coarse cells on the fly)."""
         self = cls(None)
         sets = (L.PatchSet * len(scenes))(*[L.patch_set(sc) for sc in scenes])
         self._keep = scenes
         h = C.c_void_p()
-        L.check(L.lib().bbrt_scene_create_batch(sets, C.c_int32(len(scenes)), C.byref(h)), "bbrt_scene_create_batch")
+        L.check(L.lib().bbrt_scene_create_ex(sets, C.c_int32(len(scenes)), C.c_uint32(L.SCENE_COMPACT if compact else 0),
+                                             C.byref(h)), "bbrt_scene_create_ex")
         self.h = h
         self.n_scenes = len(scenes)
         self.patch_begin = np.zeros(self.n_scenes + 1, np.int64)

@@ -165,6 +165,14 @@ int bbrt_trace_device(bbrt_scene scene, const double* origins, const double* dir
  * patch_begin[i] .. patch_begin[i+1]-1.  No counterpart in the reference,
  * whose drivers trace one scene at a time (RayTracingDriver.hpp:119-150). */
 int bbrt_scene_create_batch(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out);
+/* bbrt_scene_create_batch with scene flags.  BBRT_SCENE_COMPACT keeps neither
+ * the 100x100 coarse grid, the cell sagittas and normals nor the FP32 cell
+ * boxes (~800 KB per patch): the kernels evaluate each coarse cell a ray
+ * reaches from the nodal fields and small per-node-set tables, bit-identical
+ * to the stored values, so records and counters are the same.  About 33 KB per
+ * order-4 patch stays resident.  bbrt_scene_coarse is unavailable on it. */
+enum { BBRT_SCENE_COMPACT = 1u };
+int bbrt_scene_create_ex(const bbrt_patch_set* sets, int32_t n_sets, uint32_t scene_flags, bbrt_scene* out);
 /* Sub-scene count, patch_begin [n+1], BVH root node [n], diagonal [n]; any pointer may be NULL. */
 int bbrt_scene_batch_info(bbrt_scene scene, int32_t* n_scenes, int64_t* patch_begin, int32_t* bvh_root,
                           double* diag);

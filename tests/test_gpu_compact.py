@@ -1,0 +1,62 @@
+"""Compact scenes (BBRT_SCENE_COMPACT): coarse cells evaluated on the fly from the nodal fields.
+
+The stored and compact scenes must give identical records and identical work counters: the on-the-fly
+corners, sagittas, normals and FP32 cell boxes are bit-identical to the stored ones, so the same cells pass
+the same tests.
+"""
+import numpy as np
+import pytest
+
+from billardballraytracing3d_amd import _lib as L
+from billardballraytracing3d_amd import scenes as S
+from billardballraytracing3d_amd import tracer as T
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("status", "err_kind", "bounces", "err_patch", "hit_patch", "hit_xyz", "iters", "final_od")
+
+
+def _same(a, b):
+    for f in FIELDS:
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    np.testing.assert_array_equal(a.counters[:10], b.counters[:10])
+
+
+@pytest.mark.parametrize("mode", [L.NEWTON_REF, L.NEWTON_CONVERGED])
+def test_compact_equals_stored_config1(mode):
+    sc, o, d, mb = S.config1()
+    st = T.GpuScene(sc)
+    cp = T.GpuScene.batch([sc], compact=True)
+    _same(cp.trace(o, d, mb, mode=mode), st.trace(o, d, mb, mode=mode))
+    assert cp.info()["device_bytes"] * 10 < st.info()["device_bytes"]
+
+
+def test_compact_equals_stored_sinai_sample():
+    sc, o, d, mb = S.config2(0, 1 << 20)
+    sel = slice(0, 1 << 20, 512)
+    st = T.GpuScene(sc)
+    cp = T.GpuScene.batch([sc], compact=True)
+    a = cp.trace(o[sel], d[sel], mb, mode=L.NEWTON_CONVERGED)
+    _same(a, st.trace(o[sel], d[sel], mb, mode=L.NEWTON_CONVERGED))
+    assert a.counter("bounces") > 0
+
+
+def test_compact_batch_equals_stored_batch():
+    scenes, o, d, rb = S.batch_config(seed=5, n_scenes=5, patch_lo=80, patch_hi=600, rays_lo=300, rays_hi=2000)
+    a = T.GpuScene.batch(scenes, compact=True).trace(o, d, 25, ray_begin=rb)
+    b = T.GpuScene.batch(scenes).trace(o, d, 25, ray_begin=rb)
+    _same(a, b)
+
+
+def test_compact_brute_force_equals_stored():
+    sc, o, d, mb = S.config1(nx=8, ny=8)
+    st = T.GpuScene(sc)
+    cp = T.GpuScene.batch([sc], compact=True)
+    _same(cp.trace(o, d, 3, flags=L.FLAG_NO_CULL), st.trace(o, d, 3, flags=L.FLAG_NO_CULL))
+
+
+def test_compact_scene_has_no_coarse_grid():
+    sc, _, _, _ = S.config1(nx=2, ny=2)
+    cp = T.GpuScene.batch([sc], compact=True)
+    with pytest.raises(L.BBRTError, match="compact"):
+        cp.coarse(0)
