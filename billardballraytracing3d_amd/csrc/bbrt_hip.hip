@@ -336,6 +336,147 @@ __global__ void k_zero_i32(int32_t* p, int n) {
 
 namespace {
 
+// ---------------------------------------------------------------------------
+// Device build of compact scenes (bit-identical to scene_build.cpp)
+// ---------------------------------------------------------------------------
+// Pass 1, one workgroup per patch: max sagitta, finiteness of every coarse
+// value, longest coarse-cell diagonal (prepare_patch's statistics).
+__global__ void __launch_bounds__(256) k_patch_stats(DevScene S, double* __restrict__ max_sag, int32_t* __restrict__ finite,
+                                                     double* __restrict__ hmax) {
+    __shared__ double red_ms[256], red_h[256];
+    __shared__ int red_f[256];
+    const int64_t k = blockIdx.x;
+    double ms = 0.0, h = 0.0;
+    bool fin = true;
+    for (int c = threadIdx.x; c < bbrt::kCells * bbrt::kCells; c += blockDim.x) {
+        const int i = c / bbrt::kCells, j = c % bbrt::kCells;
+        const CellData cd = compact_cell_any(S, k, i, j);
+        ms = (ms < cd.s) ? cd.s : ms;
+        const V3* P[4] = { &cd.p00, &cd.p10, &cd.p01, &cd.p11 };
+        bool f = isfinite(cd.s) && isfinite(cd.n.x) && isfinite(cd.n.y) && isfinite(cd.n.z);
+        for (int q = 0; q < 4; ++q) f = f && isfinite(P[q]->x) && isfinite(P[q]->y) && isfinite(P[q]->z);
+        fin = fin && f;
+        const double ax = cd.p11.x - cd.p00.x, ay = cd.p11.y - cd.p00.y, az = cd.p11.z - cd.p00.z;
+        const double ex = cd.p10.x - cd.p01.x, ey = cd.p10.y - cd.p01.y, ez = cd.p10.z - cd.p01.z;
+        double a = 0.0, e = 0.0;
+        a += ax * ax; a += ay * ay; a += az * az;
+        e += ex * ex; e += ey * ey; e += ez * ez;
+        const double hc = sqrt((a < e) ? e : a);
+        h = (h < hc) ? hc : h;
+    }
+    red_ms[threadIdx.x] = ms; red_h[threadIdx.x] = h; red_f[threadIdx.x] = fin ? 1 : 0;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            const double a = red_ms[threadIdx.x + off], b = red_h[threadIdx.x + off];
+            red_ms[threadIdx.x] = (red_ms[threadIdx.x] < a) ? a : red_ms[threadIdx.x];
+            red_h[threadIdx.x] = (red_h[threadIdx.x] < b) ? b : red_h[threadIdx.x];
+            red_f[threadIdx.x] &= red_f[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        max_sag[k] = red_ms[0];
+        finite[k] = red_f[0];
+        hmax[k] = red_f[0] ? red_h[0] : 0.0;
+    }
+}
+
+// Pass 2, one workgroup per patch of one sub-scene: the FP32 pyramid in the
+// grouped layout (build_pyramid without the cell level, which compact scenes
+// rebuild on the fly).
+__global__ void __launch_bounds__(256) k_patch_pyramid(DevScene S, int64_t p0, double widen, float* __restrict__ box_tile,
+                                                       float* __restrict__ box_block, float* __restrict__ box_super,
+                                                       float* __restrict__ box_patch, uint8_t* __restrict__ qbox) {
+    __shared__ float tb[bbrt::kTileN * bbrt::kTileN][6];
+    __shared__ float bb[bbrt::kBlockN * bbrt::kBlockN][6];
+    __shared__ float sb[bbrt::kSuperN * bbrt::kSuperN][6];
+    const int64_t k = p0 + blockIdx.x;
+    const int tid = threadIdx.x;
+    const bool cull = S.meta[k].cull != 0;
+    float* gt = box_tile + (size_t)k * bbrt::kTileFloats;
+    float* gb = box_block + (size_t)k * bbrt::kBlockFloats;
+    float* gs = box_super + (size_t)k * bbrt::kSuperFloats;
+    float* gp = box_patch + (size_t)k * 8;
+    auto merge = [](float* dst, const float* src) {
+        for (int a = 0; a < 3; ++a) { dst[a] = (src[a] < dst[a]) ? src[a] : dst[a]; dst[3 + a] = (dst[3 + a] < src[3 + a]) ? src[3 + a] : dst[3 + a]; }
+    };
+    auto empty = [](float* b) { b[0] = b[1] = b[2] = INFINITY; b[3] = b[4] = b[5] = -INFINITY; };
+    if (cull) {
+        for (int t = tid; t < bbrt::kTileN * bbrt::kTileN; t += blockDim.x) {
+            const int ti = t / bbrt::kTileN, tj = t % bbrt::kTileN;
+            float b[6];
+            float cb[9][6];
+            empty(b);
+            for (int cq = 0; cq < 9; ++cq) {
+                const CellData cd = compact_cell_any(S, k, 3 * ti + cq / 3, 3 * tj + cq % 3);
+                compact_cell_box_f(cd, widen, cb[cq]);
+                merge(b, cb[cq]);
+            }
+            for (int a = 0; a < 6; ++a) tb[t][a] = b[a];
+            uint8_t* q = qbox + (size_t)k * bbrt::kQCellPatchBytes + (size_t)t * bbrt::kQCellBytes;
+            for (int cq = 0; cq < 9; ++cq) bbrt::quantize_cell_box(b, cb[cq], q + 6 * cq);
+            q[54] = 0; q[55] = 0;
+        }
+        __syncthreads();
+        for (int q = tid; q < bbrt::kBlockN * bbrt::kBlockN; q += blockDim.x) {
+            const int bi = q / bbrt::kBlockN, bj = q % bbrt::kBlockN;
+            float b[6];
+            empty(b);
+            for (int ti = 3 * bi; ti < 3 * bi + 3; ++ti)
+                for (int tj = 3 * bj; tj < 3 * bj + 3; ++tj) merge(b, tb[ti * bbrt::kTileN + tj]);
+            for (int a = 0; a < 6; ++a) bb[q][a] = b[a];
+        }
+        __syncthreads();
+        if (tid < bbrt::kSuperN * bbrt::kSuperN) {
+            const int si = tid / bbrt::kSuperN, sj = tid % bbrt::kSuperN;
+            float b[6];
+            empty(b);
+            for (int bi = bbrt::kBlocksPerSuper * si; bi < min(bbrt::kBlockN, bbrt::kBlocksPerSuper * (si + 1)); ++bi)
+                for (int bj = bbrt::kBlocksPerSuper * sj; bj < min(bbrt::kBlockN, bbrt::kBlocksPerSuper * (sj + 1)); ++bj)
+                    merge(b, bb[bi * bbrt::kBlockN + bj]);
+            for (int a = 0; a < 6; ++a) sb[tid][a] = b[a];
+        }
+        __syncthreads();
+    }
+    // tile groups: by block, slot 3*ti' + tj'; two zero pad floats per group
+    for (int q = tid; q < (int)bbrt::kTileFloats; q += blockDim.x) {
+        const int g = q / bbrt::kG9, w = q % bbrt::kG9;
+        float v = 0.0f;
+        if (w < 54) {
+            const int slot = w / 6, a = w % 6;
+            const int ti = 3 * (g / bbrt::kBlockN) + slot / 3, tj = 3 * (g % bbrt::kBlockN) + slot % 3;
+            v = cull ? tb[ti * bbrt::kTileN + tj][a] : (a < 3 ? INFINITY : -INFINITY);
+        }
+        gt[q] = v;
+    }
+    // block groups: by super, slot 4*bi' + bj'; empty where the grid ends
+    for (int q = tid; q < (int)bbrt::kBlockFloats; q += blockDim.x) {
+        const int g = q / bbrt::kG16, w = q % bbrt::kG16;
+        const int slot = w / 6, a = w % 6;
+        const int bi = bbrt::kBlocksPerSuper * (g / bbrt::kSuperN) + slot / bbrt::kBlocksPerSuper;
+        const int bj = bbrt::kBlocksPerSuper * (g % bbrt::kSuperN) + slot % bbrt::kBlocksPerSuper;
+        gb[q] = (cull && bi < bbrt::kBlockN && bj < bbrt::kBlockN) ? bb[bi * bbrt::kBlockN + bj][a]
+                                                                 : (a < 3 ? INFINITY : -INFINITY);
+    }
+    for (int q = tid; q < (int)bbrt::kSuperFloats; q += blockDim.x) {
+        float v = 0.0f;
+        if (q < 54) v = cull ? sb[q / 6][q % 6] : ((q % 6) < 3 ? INFINITY : -INFINITY);
+        gs[q] = v;
+    }
+    if (tid == 0) {
+        float b[6];
+        if (cull) {
+            empty(b);
+            for (int q = 0; q < bbrt::kSuperN * bbrt::kSuperN; ++q) merge(b, sb[q]);
+        } else {   // full box: always entered (such a patch is tested exhaustively)
+            b[0] = b[1] = b[2] = -INFINITY; b[3] = b[4] = b[5] = INFINITY;
+        }
+        for (int a = 0; a < 6; ++a) gp[a] = b[a];
+        gp[6] = 0.0f; gp[7] = 0.0f;
+    }
+}
+
 // Per-ray sub-scene ids from the sub-scene ray ranges (binary search).
 __global__ void k_ray_scene(const int64_t* __restrict__ begin, int n_scenes, int64_t n, int32_t* __restrict__ out) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -542,43 +683,46 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     }
 }
 
-// Builds every sub-scene on the host (in parallel over sub-scenes when there
-// are many) and uploads them as one device scene: patches, coarse data and
-// culling boxes concatenated in sub-scene order, each sub-scene's BVH appended
-// with its node, leaf and patch indices offset to global ones.
-int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, const char* who, bool compact) {
+// Runs fn(i) for i in [0, n) on up to `threads` host threads; rethrows the first error.
+template <class F>
+void for_each_parallel(int32_t n, unsigned threads, F&& fn, const char* who) {
+    std::atomic<int32_t> next{ 0 };
+    std::vector<std::string> errs((size_t)n);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < std::max(1u, std::min<unsigned>(threads, (unsigned)n)); ++t)
+        pool.emplace_back([&]() {
+            for (int32_t i = next++; i < n; i = next++) {
+                try { fn(i); } catch (const std::exception& e) { errs[i] = e.what(); }
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (int32_t i = 0; i < n; ++i)
+        if (!errs[i].empty()) throw std::runtime_error(std::string(who) + ": sub-scene " + std::to_string(i) + ": " + errs[i]);
+}
+
+// Builds every sub-scene and uploads them as one device scene: patches, coarse
+// data and culling boxes concatenated in sub-scene order, each sub-scene's BVH
+// appended with its node, leaf and patch indices offset to global ones.
+// gpu_build (compact scenes): the host computes only patch records, node-set
+// tables, centroids and node boxes; k_patch_stats computes each patch's max
+// sagitta, finiteness and longest cell diagonal, the host then inflates the
+// boxes and builds the BVHs, and k_patch_pyramid builds the FP32 pyramid.
+int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, const char* who, bool compact,
+                  bool gpu_build) {
     *out = nullptr;
     bbrt_scene s = nullptr;
     try {
         int ndev = 0;
         check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
         if (ndev <= 0) throw std::runtime_error(std::string(who) + ": no HIP device");
+        if (gpu_build && !compact) throw std::runtime_error(std::string(who) + ": the device build is for compact scenes");
         const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::vector<bbrt::HostScene> H((size_t)n_sets);
-        if (n_sets == 1) {
-            bbrt::build_host_scene(sets[0], H[0], (int)hw, compact);
-        } else {
-            // sub-scenes spread over the host threads, each built single-threaded
-            std::atomic<int32_t> next{ 0 };
-            std::vector<std::string> errs((size_t)n_sets);
-            std::vector<std::thread> pool;
-            for (unsigned t = 0; t < hw; ++t)
-                pool.emplace_back([&]() {
-                    for (int32_t i = next++; i < n_sets; i = next++) {
-                        try { bbrt::build_host_scene(sets[i], H[i], 1, compact); }
-                        catch (const std::exception& e) { errs[i] = e.what(); }
-                    }
-                });
-            for (auto& th : pool) th.join();
-            for (int32_t i = 0; i < n_sets; ++i)
-                if (!errs[i].empty()) throw std::runtime_error(std::string(who) + ": sub-scene " + std::to_string(i) + ": " + errs[i]);
-        }
-        // totals and offsets
-        int64_t n_patch = 0, n_node = 0, n_field = 0, n_bvh = 0, n_leaf = 0;
-        for (const auto& h : H) { n_patch += h.n; n_node += (int64_t)h.nodes.size(); n_field += (int64_t)h.fields.size();
-                                  n_bvh += (int64_t)h.bvh.size(); n_leaf += (int64_t)h.leaf.size(); }
-        if (n_patch >= (int64_t)1 << 27 || n_bvh >= (int64_t)1 << 28)
-            throw std::runtime_error(std::string(who) + ": too many patches or BVH nodes for one scene handle");
+        if (n_sets == 1) bbrt::build_host_scene(sets[0], H[0], (int)hw, compact, gpu_build);
+        else for_each_parallel(n_sets, hw, [&](int32_t i) { bbrt::build_host_scene(sets[i], H[i], 1, compact, gpu_build); }, who);
+        int64_t n_patch = 0;
+        for (const auto& h : H) n_patch += h.n;
+        if (n_patch >= (int64_t)1 << 27) throw std::runtime_error(std::string(who) + ": too many patches for one scene handle");
         s = new bbrt_scene_s();
         check(hipGetDevice(&s->device), "hipGetDevice");
         check(hipDeviceGetAttribute(&s->n_cu, hipDeviceAttributeMultiprocessorCount, s->device), "CU count");
@@ -589,48 +733,38 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
         check(hipEventCreate(&s->ev[1]), "hipEventCreate");
         check(hipHostMalloc(reinterpret_cast<void**>(&s->h_count), sizeof(int32_t) * 64, hipHostMallocDefault),
               "hipHostMalloc");
-        std::vector<PatchMeta> meta;
-        meta.reserve((size_t)n_patch);
-        std::vector<BvhNode> bvh;
-        bvh.reserve((size_t)n_bvh);
-        std::vector<int32_t> leaf;
-        leaf.reserve((size_t)n_leaf);
         s->n_scenes = n_sets;
         s->compact = compact;
         s->patch_begin.assign(1, 0);
-        int64_t po = 0, no = 0, fo = 0;
-        int32_t so = 0;   // node-set offset of the sub-scene's tables
-        std::vector<double> ff_rows, mid_basis;
-        for (int32_t i = 0; i < n_sets; ++i) {
-            const auto& h = H[i];
-            const int32_t bo = (int32_t)bvh.size(), lo = (int32_t)leaf.size();
-            if (compact && so + h.n_nodesets >= (1 << 16))
-                throw std::runtime_error(std::string(who) + ": more than 65535 distinct node sets");
-            for (PatchMeta m : h.meta) {
-                m.node_off += no; m.field_off += fo;
-                if (compact) m.nodeset = (((m.nodeset >> 16) + so) << 16) | ((m.nodeset & 0xFFFF) + so);
-                meta.push_back(m);
+        for (const auto& h : H) s->patch_begin.push_back(s->patch_begin.back() + h.n);
+
+        // patch records with global node, field and node-set offsets (fixed before the
+        // host arrays are released by the uploads)
+        std::vector<int64_t> node_base((size_t)n_sets), field_base((size_t)n_sets);
+        std::vector<int32_t> set_base((size_t)n_sets);
+        {
+            int64_t no = 0, fo = 0;
+            int32_t so = 0;
+            for (int32_t i = 0; i < n_sets; ++i) {
+                node_base[i] = no; field_base[i] = fo; set_base[i] = so;
+                if (compact && so + H[i].n_nodesets >= (1 << 16))
+                    throw std::runtime_error(std::string(who) + ": more than 65535 distinct node sets");
+                no += (int64_t)H[i].nodes.size(); fo += (int64_t)H[i].fields.size(); so += H[i].n_nodesets;
             }
-            if (compact) {
-                ff_rows.insert(ff_rows.end(), h.ff_rows.begin(), h.ff_rows.end());
-                mid_basis.insert(mid_basis.end(), h.mid_basis.begin(), h.mid_basis.end());
-                so += h.n_nodesets;
-            }
-            for (BvhNode nd : h.bvh) {
-                if (nd.right >= 0) nd.right += bo; else nd.start += lo;
-                bvh.push_back(nd);
-            }
-            for (int32_t o : h.leaf) leaf.push_back(o + (int32_t)po);
-            bbrt::SceneInfo si{};
-            si.root = bo; si.patch_begin = (int32_t)po; si.diag = h.diag; si.cull_limit = 8.0 * h.scale;
-            si.cull_widen = h.cull_widen;
-            s->scenes.push_back(si);
-            po += h.n; no += (int64_t)h.nodes.size(); fo += (int64_t)h.fields.size();
-            s->patch_begin.push_back(po);
         }
-        for (const BvhNode& nd : bvh)
-            if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
-                throw std::runtime_error(std::string(who) + ": BVH exceeds the packed node limits");
+        std::vector<PatchMeta> meta;
+        auto merge_meta = [&]() {
+            meta.clear();
+            meta.reserve((size_t)n_patch);
+            for (int32_t i = 0; i < n_sets; ++i) {
+                const int32_t so = set_base[i];
+                for (PatchMeta m : H[i].meta) {
+                    m.node_off += node_base[i]; m.field_off += field_base[i];
+                    if (compact) m.nodeset = (((m.nodeset >> 16) + so) << 16) | ((m.nodeset & 0xFFFF) + so);
+                    meta.push_back(m);
+                }
+            }
+        };
         // big arrays go straight from each sub-scene to its slice of the device array
         auto upload_parts = [&](auto member) {
             using T = typename std::decay_t<decltype(H[0].*member)>::value_type;
@@ -647,27 +781,103 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
                 at += v.size();
                 std::vector<T>().swap(v);   // release host memory as it goes
             }
-            return static_cast<const T*>(d);
+            return d;
         };
         DevScene& D = s->dev;
-        D.meta = dev_upload(meta, s->allocs, s->bytes);
+        merge_meta();
+        PatchMeta* d_meta = const_cast<PatchMeta*>(dev_upload(meta, s->allocs, s->bytes));
+        D.meta = d_meta;
+        D.ff_rows = compact ? upload_parts(&bbrt::HostScene::ff_rows) : nullptr;
+        D.mid_basis = compact ? upload_parts(&bbrt::HostScene::mid_basis) : nullptr;
+        D.compact = compact ? 1 : 0;
         D.nodes = upload_parts(&bbrt::HostScene::nodes);
         D.fields = upload_parts(&bbrt::HostScene::fields);
         D.cxyz = upload_parts(&bbrt::HostScene::cxyz);
         D.cell = upload_parts(&bbrt::HostScene::cell);
         D.box_cell = upload_parts(&bbrt::HostScene::box_cell);
-        D.box_tile = upload_parts(&bbrt::HostScene::box_tile);
-        D.box_block = upload_parts(&bbrt::HostScene::box_block);
-        D.box_super = upload_parts(&bbrt::HostScene::box_super);
-        D.box_patch = upload_parts(&bbrt::HostScene::box_patch);
+        D.ray_scene = nullptr;
+        if (gpu_build) {
+            // pass 1 on the device, then the host side of every sub-scene
+            std::vector<double> ms((size_t)n_patch), hm((size_t)n_patch);
+            std::vector<int32_t> fin((size_t)n_patch);
+            double* d_ms = nullptr;
+            double* d_hm = nullptr;
+            int32_t* d_fin = nullptr;
+            check(hipMalloc(&d_ms, n_patch * sizeof(double)), "hipMalloc");
+            check(hipMalloc(&d_hm, n_patch * sizeof(double)), "hipMalloc");
+            check(hipMalloc(&d_fin, n_patch * sizeof(int32_t)), "hipMalloc");
+            hipLaunchKernelGGL(k_patch_stats, dim3((unsigned)n_patch), dim3(256), 0, s->stream, D, d_ms, d_fin, d_hm);
+            check(hipGetLastError(), "k_patch_stats");
+            check(hipMemcpyAsync(ms.data(), d_ms, n_patch * sizeof(double), hipMemcpyDeviceToHost, s->stream), "D2H");
+            check(hipMemcpyAsync(hm.data(), d_hm, n_patch * sizeof(double), hipMemcpyDeviceToHost, s->stream), "D2H");
+            check(hipMemcpyAsync(fin.data(), d_fin, n_patch * sizeof(int32_t), hipMemcpyDeviceToHost, s->stream), "D2H");
+            check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+            (void)hipFree(d_ms); (void)hipFree(d_hm); (void)hipFree(d_fin);
+            for_each_parallel(n_sets, hw, [&](int32_t i) {
+                const int64_t p0 = s->patch_begin[i], pn = H[i].n;
+                bbrt::finish_host_scene(H[i], std::vector<double>(ms.begin() + p0, ms.begin() + p0 + pn),
+                                        std::vector<int32_t>(fin.begin() + p0, fin.begin() + p0 + pn),
+                                        std::vector<double>(hm.begin() + p0, hm.begin() + p0 + pn));
+            }, who);
+            merge_meta();
+            check(hipMemcpy(d_meta, meta.data(), meta.size() * sizeof(PatchMeta), hipMemcpyHostToDevice), "H2D meta");
+        }
+        // BVHs, leaves and the sub-scene table
+        std::vector<BvhNode> bvh;
+        std::vector<int32_t> leaf;
+        for (int32_t i = 0; i < n_sets; ++i) {
+            const auto& h = H[i];
+            const int32_t bo = (int32_t)bvh.size(), lo = (int32_t)leaf.size();
+            for (BvhNode nd : h.bvh) {
+                if (nd.right >= 0) nd.right += bo; else nd.start += lo;
+                bvh.push_back(nd);
+            }
+            for (int32_t o : h.leaf) leaf.push_back(o + (int32_t)s->patch_begin[i]);
+            bbrt::SceneInfo si{};
+            si.root = bo; si.patch_begin = (int32_t)s->patch_begin[i]; si.diag = h.diag; si.cull_limit = 8.0 * h.scale;
+            si.cull_widen = h.cull_widen;
+            s->scenes.push_back(si);
+        }
+        if ((int64_t)bvh.size() >= (int64_t)1 << 28) throw std::runtime_error(std::string(who) + ": too many BVH nodes");
+        for (const BvhNode& nd : bvh)
+            if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
+                throw std::runtime_error(std::string(who) + ": BVH exceeds the packed node limits");
         D.bvh = dev_upload(bvh, s->allocs, s->bytes);
         D.leaf = dev_upload(leaf, s->allocs, s->bytes);
         s->d_scenes = const_cast<bbrt::SceneInfo*>(dev_upload(s->scenes, s->allocs, s->bytes));
         D.scenes = s->d_scenes;
-        D.ray_scene = nullptr;
-        D.ff_rows = compact ? dev_upload(ff_rows, s->allocs, s->bytes) : nullptr;
-        D.mid_basis = compact ? dev_upload(mid_basis, s->allocs, s->bytes) : nullptr;
-        D.compact = compact ? 1 : 0;
+        if (gpu_build) {
+            auto dalloc = [&](size_t floats) {
+                float* d = nullptr;
+                check(hipMalloc(&d, std::max<size_t>(1, floats) * sizeof(float)), "hipMalloc(scene)");
+                s->allocs.push_back(d);
+                s->bytes += (int64_t)(std::max<size_t>(1, floats) * sizeof(float));
+                return d;
+            };
+            float* bt = dalloc((size_t)n_patch * bbrt::kTileFloats);
+            float* bk = dalloc((size_t)n_patch * bbrt::kBlockFloats);
+            float* bs = dalloc((size_t)n_patch * bbrt::kSuperFloats);
+            float* bp = dalloc((size_t)n_patch * 8);
+            uint8_t* qb = nullptr;
+            check(hipMalloc(&qb, std::max<size_t>(1, (size_t)n_patch * bbrt::kQCellPatchBytes)), "hipMalloc(scene)");
+            s->allocs.push_back(qb);
+            s->bytes += (int64_t)n_patch * (int64_t)bbrt::kQCellPatchBytes;
+            // (the quantized boxes of a patch without culling are never read and stay unwritten)
+            for (int32_t i = 0; i < n_sets; ++i)
+                if (H[i].n > 0)
+                    hipLaunchKernelGGL(k_patch_pyramid, dim3((unsigned)H[i].n), dim3(256), 0, s->stream, D,
+                                       s->patch_begin[i], H[i].cull_widen, bt, bk, bs, bp, qb);
+            D.qbox_cell = qb;
+            check(hipGetLastError(), "k_patch_pyramid");
+            check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+            D.box_tile = bt; D.box_block = bk; D.box_super = bs; D.box_patch = bp;
+        } else {
+            D.qbox_cell = compact ? upload_parts(&bbrt::HostScene::qbox_cell) : nullptr;
+            D.box_tile = upload_parts(&bbrt::HostScene::box_tile);
+            D.box_block = upload_parts(&bbrt::HostScene::box_block);
+            D.box_super = upload_parts(&bbrt::HostScene::box_super);
+            D.box_patch = upload_parts(&bbrt::HostScene::box_patch);
+        }
         s->n_patches = n_patch;
         for (auto& h : H) {
             s->pbox.insert(s->pbox.end(), h.pbox.begin(), h.pbox.end());
@@ -776,20 +986,28 @@ int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int
 
 int bbrt_scene_create(const bbrt_patch_set* ps, bbrt_scene* out) {
     if (!ps || !out) { g_last_error = "bbrt_scene_create: NULL argument"; return -1; }
-    return create_scenes(ps, 1, out, "bbrt_scene_create", false);
+    return create_scenes(ps, 1, out, "bbrt_scene_create", false, false);
 }
 
 int bbrt_scene_create_batch(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out) {
     if (!sets || !out) { g_last_error = "bbrt_scene_create_batch: NULL argument"; return -1; }
     if (n_sets <= 0) { g_last_error = "bbrt_scene_create_batch: no sub-scenes"; return -1; }
-    return create_scenes(sets, n_sets, out, "bbrt_scene_create_batch", false);
+    return create_scenes(sets, n_sets, out, "bbrt_scene_create_batch", false, false);
 }
 
 int bbrt_scene_create_ex(const bbrt_patch_set* sets, int32_t n_sets, uint32_t scene_flags, bbrt_scene* out) {
     if (!sets || !out) { g_last_error = "bbrt_scene_create_ex: NULL argument"; return -1; }
     if (n_sets <= 0) { g_last_error = "bbrt_scene_create_ex: no sub-scenes"; return -1; }
-    if (scene_flags & ~(uint32_t)BBRT_SCENE_COMPACT) { g_last_error = "bbrt_scene_create_ex: unknown flag"; return -1; }
-    return create_scenes(sets, n_sets, out, "bbrt_scene_create_ex", (scene_flags & BBRT_SCENE_COMPACT) != 0);
+    if (scene_flags & ~(uint32_t)(BBRT_SCENE_COMPACT | BBRT_SCENE_DEVICE_BUILD)) {
+        g_last_error = "bbrt_scene_create_ex: unknown flag";
+        return -1;
+    }
+    if ((scene_flags & BBRT_SCENE_DEVICE_BUILD) && !(scene_flags & BBRT_SCENE_COMPACT)) {
+        g_last_error = "bbrt_scene_create_ex: BBRT_SCENE_DEVICE_BUILD needs BBRT_SCENE_COMPACT";
+        return -1;
+    }
+    return create_scenes(sets, n_sets, out, "bbrt_scene_create_ex", (scene_flags & BBRT_SCENE_COMPACT) != 0,
+                         (scene_flags & BBRT_SCENE_DEVICE_BUILD) != 0);
 }
 
 int bbrt_scene_batch_info(bbrt_scene s, int32_t* n_scenes, int64_t* patch_begin, int32_t* bvh_root, double* diag) {

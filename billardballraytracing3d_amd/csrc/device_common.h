@@ -46,6 +46,7 @@ struct DevScene {
     // compact scenes: no cxyz / cell / box_cell; cells are evaluated from the node-set tables
     const double* ff_rows;           // [set][kCoarseN][kRowStride]
     const double* mid_basis;         // [set][kCells][kMaxNodes1D]
+    const uint8_t* qbox_cell;        // [n][kQCellPatchBytes]
     int compact;
 };
 
@@ -195,59 +196,85 @@ struct CellData {
 
 // Coarse vertex (i, j): (sum_m (sum_q ru[i][q] f[q][m]) rv[j][m]) * (su[i] sv[j]) per coordinate,
 // first-form interpolation (PatchInterpolation.hpp:54-159) with the inner index summed upwards.
+// N > 0: N x N nodes known at compile time.
+template <int N>
 __device__ __forceinline__ V3 coarse_point(const DevScene& S, const PatchMeta& M, int i, int j) {
-    const int im = M.imax, jm = M.jmax, nn = im * jm;
+    const int im = N > 0 ? N : M.imax, jm = N > 0 ? N : M.jmax, nn = im * jm;
     const double* ru = S.ff_rows + ((size_t)(M.nodeset >> 16) * bbrt::kCoarseN + i) * bbrt::kRowStride;
     const double* rv = S.ff_rows + ((size_t)(M.nodeset & 0xFFFF) * bbrt::kCoarseN + j) * bbrt::kRowStride;
     const double* F = S.fields + M.field_off;
     const double sc = ru[bbrt::kMaxNodes1D] * rv[bbrt::kMaxNodes1D];
+    constexpr int L = N > 0 ? N : bbrt::kMaxNodes1D;
     double out[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double* f = F + c * nn;
         double acc2 = 0.0;
-        for (int m = 0; m < jm; ++m) {
-            double acc = 0.0;
-            for (int q = 0; q < im; ++q) acc += ru[q] * f[q * jm + m];
-            acc2 += acc * rv[m];
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            if (m < jm) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < L; ++q)
+                    if (q < im) acc += ru[q] * f[q * jm + m];
+                acc2 += acc * rv[m];
+            }
         }
         out[c] = acc2 * sc;
     }
     return { out[0], out[1], out[2] };
 }
 
-__device__ __forceinline__ double field_tab(const double* lu, const double* lv, const double* f, int im, int jm) {
+template <int N>
+__device__ __forceinline__ double field_tab(const double* lu, const double* lv, const double* f, int im_, int jm_) {
+    const int im = N > 0 ? N : im_, jm = N > 0 ? N : jm_;
+    constexpr int L = N > 0 ? N : bbrt::kMaxNodes1D;
     double sum = 0.0;
-    for (int i = 0; i < im; ++i) {
-        const double li = lu[i];
-        if (li != 0.0) {
-            double row = 0.0;
-            for (int j = 0; j < jm; ++j) row += lv[j] * f[i * jm + j];
-            sum += li * row;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        if (i < im) {
+            const double li = lu[i];
+            if (li != 0.0) {
+                double row = 0.0;
+#pragma unroll
+                for (int j = 0; j < L; ++j)
+                    if (j < jm) row += lv[j] * f[i * jm + j];
+                sum += li * row;
+            }
         }
     }
     return sum;
 }
 
 // Corners, sagitta and normal of cell (i, j) (SurfacePatch.hpp:416-492).
+template <int N>
 __device__ CellData compact_cell(const DevScene& S, int64_t k, int i, int j) {
     const PatchMeta M = S.meta[k];
     CellData cd;
-    cd.p00 = coarse_point(S, M, i, j);
-    cd.p10 = coarse_point(S, M, i + 1, j);
-    cd.p01 = coarse_point(S, M, i, j + 1);
-    cd.p11 = coarse_point(S, M, i + 1, j + 1);
+    cd.p00 = coarse_point<N>(S, M, i, j);
+    cd.p10 = coarse_point<N>(S, M, i + 1, j);
+    cd.p01 = coarse_point<N>(S, M, i, j + 1);
+    cd.p11 = coarse_point<N>(S, M, i + 1, j + 1);
     const int im = M.imax, jm = M.jmax, nn = im * jm;
     const double* lu = S.mid_basis + ((size_t)(M.nodeset >> 16) * bbrt::kCells + i) * bbrt::kMaxNodes1D;
     const double* lv = S.mid_basis + ((size_t)(M.nodeset & 0xFFFF) * bbrt::kCells + j) * bbrt::kMaxNodes1D;
     const double* F = S.fields + M.field_off;
-    const V3 tm = { field_tab(lu, lv, F, im, jm), field_tab(lu, lv, F + nn, im, jm), field_tab(lu, lv, F + 2 * nn, im, jm) };
-    cd.n = unit3({ field_tab(lu, lv, F + 9 * nn, im, jm), field_tab(lu, lv, F + 10 * nn, im, jm),
-                   field_tab(lu, lv, F + 11 * nn, im, jm) });
+    const V3 tm = { field_tab<N>(lu, lv, F, im, jm), field_tab<N>(lu, lv, F + nn, im, jm),
+                    field_tab<N>(lu, lv, F + 2 * nn, im, jm) };
+    cd.n = unit3({ field_tab<N>(lu, lv, F + 9 * nn, im, jm), field_tab<N>(lu, lv, F + 10 * nn, im, jm),
+                   field_tab<N>(lu, lv, F + 11 * nn, im, jm) });
     const V3 bm = { 0.25 * (cd.p00.x + cd.p10.x + cd.p01.x + cd.p11.x), 0.25 * (cd.p00.y + cd.p10.y + cd.p01.y + cd.p11.y),
                     0.25 * (cd.p00.z + cd.p10.z + cd.p01.z + cd.p11.z) };
     const V3 dd = sub3(tm, bm);
     cd.s = sqrt(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z);
     return cd;
+}
+
+__device__ __forceinline__ CellData compact_cell_any(const DevScene& S, int64_t k, int i, int j) {
+    const PatchMeta M = S.meta[k];
+    if (M.imax == 3 && M.jmax == 3) return compact_cell<3>(S, k, i, j);
+    if (M.imax == 5 && M.jmax == 5) return compact_cell<5>(S, k, i, j);
+    return compact_cell<0>(S, k, i, j);
 }
 
 __device__ __forceinline__ float round_down_f(double v) {
@@ -261,7 +288,25 @@ __device__ __forceinline__ float round_up_f(double v) {
     return f;
 }
 
-// The cell's FP32 culling box exactly as build_pyramid makes it, and the slab test.
+// The cell's FP32 culling box exactly as build_pyramid makes it.
+__device__ __forceinline__ void compact_cell_box_f(const CellData& cd, double widen, float b[6]) {
+    double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    const double signs[3] = { 0.0, 1.0, -1.0 };
+    const V3* corner[4] = { &cd.p00, &cd.p10, &cd.p01, &cd.p11 };
+    for (int si = 0; si < 3; ++si) {
+        const double dd = signs[si] * cd.s;
+        for (int q = 0; q < 4; ++q) {
+            const double x = corner[q]->x + cd.n.x * dd, y = corner[q]->y + cd.n.y * dd, z = corner[q]->z + cd.n.z * dd;
+            lo[0] = (x < lo[0]) ? x : lo[0]; hi[0] = (hi[0] < x) ? x : hi[0];
+            lo[1] = (y < lo[1]) ? y : lo[1]; hi[1] = (hi[1] < y) ? y : hi[1];
+            lo[2] = (z < lo[2]) ? z : lo[2]; hi[2] = (hi[2] < z) ? z : hi[2];
+        }
+    }
+    b[0] = round_down_f(lo[0] - widen); b[1] = round_down_f(lo[1] - widen); b[2] = round_down_f(lo[2] - widen);
+    b[3] = round_up_f(hi[0] + widen); b[4] = round_up_f(hi[1] + widen); b[5] = round_up_f(hi[2] + widen);
+}
+
+// ... and the slab test against it.
 __device__ __forceinline__ bool compact_cell_box(const CellData& cd, double widen, const Ray32& r) {
     double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
     const double signs[3] = { 0.0, 1.0, -1.0 };
@@ -308,7 +353,7 @@ __device__ __forceinline__ void cell_tests_data(const CellData& cd, int i, int j
 __device__ void stage1_brute_compact(const DevScene& S, int64_t k, const V3& o, const V3& d, Stage1& best, Counters& c) {
     for (int i = 0; i < bbrt::kCells; ++i)
         for (int j = 0; j < bbrt::kCells; ++j) {
-            const CellData cd = compact_cell(S, k, i, j);
+            const CellData cd = compact_cell_any(S, k, i, j);
             cell_tests_data(cd, i, j, o, d, best, c);
         }
 }
@@ -363,13 +408,39 @@ __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, con
                 mt &= mt - 1;
                 const int ti = 3 * bi + tq / 3, tj = 3 * bj + tq % 3;
                 if (S.compact) {
-                    // the tile's nine cells evaluated from the fields; each box rebuilt and tested as stored
+                    // the quantized boxes (supersets) select cells; those are evaluated from the
+                    // fields, their exact box rebuilt and tested as stored, then Moller-Trumbore
+                    const float* tb = gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9 + 6 * tq;
+                    const uint2* q8 = reinterpret_cast<const uint2*>(
+                        S.qbox_cell + (size_t)k * bbrt::kQCellPatchBytes + (size_t)(ti * bbrt::kTileN + tj) * bbrt::kQCellBytes);
+                    uint8_t qb[56];
+#pragma unroll
+                    for (int w = 0; w < 7; ++w) {
+                        const uint2 v = q8[w];
+                        for (int b8 = 0; b8 < 4; ++b8) { qb[8 * w + b8] = (uint8_t)(v.x >> (8 * b8)); qb[8 * w + 4 + b8] = (uint8_t)(v.y >> (8 * b8)); }
+                    }
+                    const double tlo[3] = { tb[0], tb[1], tb[2] };
+                    const double step[3] = { ((double)tb[3] - tlo[0]) / 255.0, ((double)tb[4] - tlo[1]) / 255.0,
+                                             ((double)tb[5] - tlo[2]) / 255.0 };
+                    uint32_t mc = 0;
                     for (int cq = 0; cq < 9; ++cq) {
-                        const int i = 3 * ti + cq / 3, j = 3 * tj + cq % 3;
-                        const CellData cd = compact_cell(S, k, i, j);
-                        if (compact_cell_box(cd, widen, r)) cell_tests_data(cd, i, j, o, d, best, c);
+                        const uint8_t* qc = qb + 6 * cq;
+                        const float bx[6] = { round_down_f(tlo[0] + qc[0] * step[0]), round_down_f(tlo[1] + qc[1] * step[1]),
+                                              round_down_f(tlo[2] + qc[2] * step[2]), round_up_f(tlo[0] + qc[3] * step[0]),
+                                              round_up_f(tlo[1] + qc[4] * step[1]), round_up_f(tlo[2] + qc[5] * step[2]) };
+                        if (box32(bx, r)) mc |= 1u << cq;
                     }
                     c.cull_tests += 9;
+                    const PatchMeta M = S.meta[k];
+                    const int nn_kind = (M.imax == 3 && M.jmax == 3) ? 3 : ((M.imax == 5 && M.jmax == 5) ? 5 : 0);
+                    while (mc) {
+                        const int cq = __ffs(mc) - 1;
+                        mc &= mc - 1;
+                        const int i = 3 * ti + cq / 3, j = 3 * tj + cq % 3;
+                        const CellData cd = nn_kind == 5 ? compact_cell<5>(S, k, i, j)
+                                                         : (nn_kind == 3 ? compact_cell<3>(S, k, i, j) : compact_cell<0>(S, k, i, j));
+                        if (compact_cell_box(cd, widen, r)) cell_tests_data(cd, i, j, o, d, best, c);
+                    }
                     continue;
                 }
                 uint32_t mc = group_mask<9, bbrt::kG9>(gc + (ti * bbrt::kTileN + tj) * bbrt::kG9, r);

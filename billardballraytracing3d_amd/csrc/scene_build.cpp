@@ -231,25 +231,14 @@ struct NodeSetTable {
     }
 };
 
-void prepare_patch(PatchWork& W, int64_t k) {
-    const bbrt_patch_set& ps = *W.ps;
-    HostScene& S = *W.S;
+// Centroid and node box of patch k (SurfacePatch.hpp:380-407); the box goes to
+// pbox uninflated.
+void centroid_box(HostScene& S, int64_t k, double lo[3], double hi[3]) {
     const PatchMeta& M = S.meta[k];
-    const int im = M.imax, jm = M.jmax;
-    const double* un = &S.nodes[M.node_off];
-    const double* uw = un + im;
-    const double* vn = uw + im;
-    const double* vw = vn + jm;
     const double* F = &S.fields[M.field_off];
-    const size_t nn = (size_t)im * jm;
-    (void)ps;
-
-    // centroid and node box (SurfacePatch.hpp:380-407)
+    const size_t nn = (size_t)M.imax * M.jmax;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    double lo[3] = { std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
-                     std::numeric_limits<double>::infinity() };
-    double hi[3] = { -std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity(),
-                     -std::numeric_limits<double>::infinity() };
+    for (int a = 0; a < 3; ++a) { lo[a] = std::numeric_limits<double>::infinity(); hi[a] = -lo[a]; }
     for (size_t q = 0; q < nn; ++q) {
         const double px = F[q], py = F[nn + q], pz = F[2 * nn + q];
         sx += px; sy += py; sz += pz;
@@ -259,17 +248,22 @@ void prepare_patch(PatchWork& W, int64_t k) {
     const double inv_n = 1.0 / static_cast<double>(nn);
     double* cent = &S.pcent[3 * k];
     cent[0] = sx * inv_n; cent[1] = sy * inv_n; cent[2] = sz * inv_n;
+    double* b = &S.pbox[6 * k];
+    for (int a = 0; a < 3; ++a) { b[a] = lo[a]; b[3 + a] = hi[a]; }
+}
+
+void prepare_patch(PatchWork& W, int64_t k) {
+    HostScene& S = *W.S;
+    double lo[3], hi[3];
+    centroid_box(S, k, lo, hi);
 
     // coarse grid (SurfacePatch.hpp:416-436)
-    double cu[kCoarseN];
-    for (int i = 0; i < kCoarseN; ++i) cu[i] = -1.0 + 2.0 * static_cast<double>(i) / (kCoarseN - 1);
     double* G = S.compact ? tls_grid() : &S.cxyz[(size_t)k * kCoarseN * kCoarseN * 3];
     compute_grid(S, k, G);
 
     // cell padding (SurfacePatch.hpp:446-492)
     double* C = S.compact ? tls_cells() : &S.cell[(size_t)k * kCells * kCells * 4];
     compute_cells(S, k, G, C);
-    (void)un; (void)uw; (void)vn; (void)vw; (void)F; (void)nn; (void)cu;
     double ms = 0.0;
     bool finite = true;
     for (size_t q = 0; q < (size_t)kCells * kCells; ++q) {
@@ -379,6 +373,11 @@ void build_pyramid(HostScene& S, int64_t k, double widen) {
                 for (int j = 3 * tj; j < 3 * tj + 3; ++j) merge(b, &cellb[((size_t)i * kCells + j) * 6]);
             float* g = gt + (size_t)((ti / 3) * kBlockN + tj / 3) * kG9 + 6 * (3 * (ti % 3) + tj % 3);
             for (int a = 0; a < 6; ++a) g[a] = b[a];
+            if (S.compact) {
+                uint8_t* q = &S.qbox_cell[(size_t)k * kQCellPatchBytes + ((size_t)ti * kTileN + tj) * kQCellBytes];
+                for (int c = 0; c < 9; ++c)
+                    quantize_cell_box(b, &cellb[((size_t)(3 * ti + c / 3) * kCells + 3 * tj + c % 3) * 6], q + 6 * c);
+            }
         }
     for (int bi = 0; bi < kBlockN; ++bi)
         for (int bj = 0; bj < kBlockN; ++bj) {
@@ -474,9 +473,55 @@ void parallel_for(int64_t n, int threads, F&& fn) {
     if (err) std::rethrow_exception(err);
 }
 
+
+// Scene diagonal, largest sagitta, scale and culling margin from the inflated
+// patch boxes and each patch's longest coarse-cell diagonal.
+void scene_constants(HostScene& S, const std::vector<double>& patch_hmax) {
+    const int64_t n = S.n;
+    // scene diagonal and largest sagitta (RayTracingDriver.hpp:386-396)
+    double lo[3] = { S.pbox[0], S.pbox[1], S.pbox[2] }, hi[3] = { S.pbox[3], S.pbox[4], S.pbox[5] };
+    double msag = 0.0, scale = 1.0, hmax = 0.0;
+    for (int64_t k = 0; k < n; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = lesser(lo[a], S.pbox[6 * k + a]);
+            hi[a] = greater(hi[a], S.pbox[6 * k + 3 + a]);
+        }
+        msag = greater(msag, S.meta[k].max_sag);
+        if (S.meta[k].cull) {
+            for (int a = 0; a < 6; ++a) scale = std::max(scale, std::fabs(S.pbox[6 * k + a]));
+            hmax = std::max(hmax, patch_hmax[k]);
+        }
+    }
+    const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    S.diag = std::sqrt(dx * dx + dy * dy + dz * dz);
+    S.max_patch_sag = msag;
+    S.scale = scale;
+    // Margin of the FP32 culling boxes.  It must exceed (a) the FP32 slab-test
+    // error for rays starting within 8*scale of the origin, about
+    // 4 * 2^-24 * 20 * scale, and (b) how far outside its triangle a hit that
+    // Moller-Trumbore accepts can lie, at most about 2^-52 * |o - v0| * h^2 / 1e-9
+    // for cell diagonal h (the 1e-9 is the determinant guard).  1e-5 * scale *
+    // max(1, h^2) exceeds both by more than an order of magnitude.
+    S.cull_widen = 1e-5 * scale * std::max(1.0, hmax * hmax);
+}
+
 }  // namespace
 
-void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads, bool compact) {
+void finish_host_scene(HostScene& S, const std::vector<double>& max_sag, const std::vector<int32_t>& finite,
+                       const std::vector<double>& hmax) {
+    for (int64_t k = 0; k < S.n; ++k) {
+        const double ms = max_sag[k];
+        double* b = &S.pbox[6 * k];
+        b[0] = b[0] - ms; b[1] = b[1] - ms; b[2] = b[2] - ms;
+        b[3] = b[3] + ms; b[4] = b[4] + ms; b[5] = b[5] + ms;
+        S.meta[k].max_sag = ms;
+        S.meta[k].cull = finite[k] ? 1 : 0;
+    }
+    scene_constants(S, hmax);
+    build_bvh(S.pbox, S.pcent, 4, S.bvh, S.leaf);
+}
+
+void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads, bool compact, bool deferred) {
     const int64_t n = ps.n_patches;
     if (n <= 0) throw std::runtime_error("bbrt_scene_create: no patches");
     if (!ps.imax || !ps.jmax || !ps.u_off || !ps.v_off || !ps.f_off || !ps.unodes || !ps.vnodes || !ps.fields)
@@ -535,36 +580,16 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads, bool 
     }
     S.pbox.assign((size_t)n * 6, 0.0);
     S.pcent.assign((size_t)n * 3, 0.0);
+    if (deferred) {   // the device computes the per-patch statistics and the pyramid (finish_host_scene)
+        parallel_for(n, threads, [&](int64_t k) { double lo[3], hi[3]; centroid_box(S, k, lo, hi); });
+        return;
+    }
     PatchWork W{ &ps, &S, std::vector<double>(n, 0.0) };
     parallel_for(n, threads, [&](int64_t k) { prepare_patch(W, k); });
-
-    // scene diagonal and largest sagitta (RayTracingDriver.hpp:386-396)
-    double lo[3] = { S.pbox[0], S.pbox[1], S.pbox[2] }, hi[3] = { S.pbox[3], S.pbox[4], S.pbox[5] };
-    double msag = 0.0, scale = 1.0, hmax = 0.0;
-    for (int64_t k = 0; k < n; ++k) {
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = lesser(lo[a], S.pbox[6 * k + a]);
-            hi[a] = greater(hi[a], S.pbox[6 * k + 3 + a]);
-        }
-        msag = greater(msag, S.meta[k].max_sag);
-        if (S.meta[k].cull) {
-            for (int a = 0; a < 6; ++a) scale = std::max(scale, std::fabs(S.pbox[6 * k + a]));
-            hmax = std::max(hmax, W.hmax[k]);
-        }
-    }
-    const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
-    S.diag = std::sqrt(dx * dx + dy * dy + dz * dz);
-    S.max_patch_sag = msag;
-    S.scale = scale;
-    // Margin of the FP32 culling boxes.  It must exceed (a) the FP32 slab-test
-    // error for rays starting within 8*scale of the origin, about
-    // 4 * 2^-24 * 20 * scale, and (b) how far outside its triangle a hit that
-    // Moller-Trumbore accepts can lie, at most about 2^-52 * |o - v0| * h^2 / 1e-9
-    // for cell diagonal h (the 1e-9 is the determinant guard).  1e-5 * scale *
-    // max(1, h^2) exceeds both by more than an order of magnitude.
-    S.cull_widen = 1e-5 * scale * std::max(1.0, hmax * hmax);
+    scene_constants(S, W.hmax);
 
     if (!compact) S.box_cell.assign((size_t)n * kCellFloats, 0.0f);
+    else S.qbox_cell.assign((size_t)n * kQCellPatchBytes, 0);
     S.box_tile.assign((size_t)n * kTileFloats, 0.0f);
     S.box_block.assign((size_t)n * kBlockFloats, 0.0f);
     S.box_super.assign((size_t)n * kSuperFloats, 0.0f);

@@ -2,6 +2,7 @@
 // (scene_build.cpp) and the device runtime (bbrt_hip.hip).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -27,6 +28,11 @@ constexpr size_t kBlockFloats = 9 * kG16;
 constexpr size_t kTileFloats = (size_t)kBlockN * kBlockN * kG9;
 constexpr size_t kCellFloats = (size_t)kTileN * kTileN * kG9;
 constexpr int kMaxNodes1D = 16;                       // largest imax / jmax the kernels accept
+// Compact scenes keep each cell's box only as 6 bytes relative to its tile's box
+// (quantized outward with one step of slack, decoded in FP64 and rounded
+// outward), a superset of the exact FP32 cell box: by tile, 9 cells x 6 bytes + 2 pad.
+constexpr int kQCellBytes = 56;
+constexpr size_t kQCellPatchBytes = (size_t)kTileN * kTileN * kQCellBytes;
 
 enum : int8_t { kEscaped = BBRT_ESCAPED, kTrapped = BBRT_TRAPPED, kError = BBRT_ERROR };
 enum : int {
@@ -94,11 +100,42 @@ struct HostScene {
     bool compact = false;
     std::vector<double> ff_rows;    // [set][kCoarseN][kRowStride]: first-form rows at the coarse values, scale
     std::vector<double> mid_basis;  // [set][kCells][kMaxNodes1D]: second-form basis at the cell midpoints
+    std::vector<uint8_t> qbox_cell; // [n][kQCellPatchBytes]: quantized cell boxes, by tile
     int n_nodesets = 0;
 };
 
+#if defined(__HIPCC__)
+#define BBRT_HOST_DEVICE __host__ __device__
+#else
+#define BBRT_HOST_DEVICE
+#endif
+
+// Quantization of a cell box b within its tile box t (both lo xyz, hi xyz) to 6
+// bytes; the host and the device build use this same code.
+BBRT_HOST_DEVICE inline void quantize_cell_box(const float* t, const float* b, uint8_t* q) {
+    for (int a = 0; a < 3; ++a) {
+        const double lo = t[a], ext = (double)t[3 + a] - (double)t[a];
+        int qlo = 0, qhi = 255;
+        if (ext > 0.0 && ext < 1e300) {
+            const double fl = ::floor(((double)b[a] - lo) / ext * 255.0) - 1.0;
+            const double fh = ::ceil(((double)b[3 + a] - lo) / ext * 255.0) + 1.0;
+            qlo = fl < 0.0 ? 0 : (fl > 255.0 ? 255 : (int)fl);
+            qhi = fh < 0.0 ? 0 : (fh > 255.0 ? 255 : (int)fh);
+        }
+        q[a] = (uint8_t)qlo;
+        q[3 + a] = (uint8_t)qhi;
+    }
+}
+
 // Builds everything above from a patch set; throws std::runtime_error on bad input.
-void build_host_scene(const bbrt_patch_set& ps, HostScene& out, int threads, bool compact = false);
+// deferred (compact scenes only): stop after the patch records, node-set
+// tables, centroids and uninflated node boxes; the device computes each
+// patch's max sagitta, finiteness and longest cell diagonal, then
+// finish_host_scene completes the host side and the device builds the pyramid.
+void build_host_scene(const bbrt_patch_set& ps, HostScene& out, int threads, bool compact = false,
+                      bool deferred = false);
+void finish_host_scene(HostScene& S, const std::vector<double>& max_sag, const std::vector<int32_t>& finite,
+                       const std::vector<double>& hmax);
 
 // BVH::build (BVH.hpp:136-189) over objects with boxes [n][6] and centroids [n][3].
 void build_bvh(const std::vector<double>& box, const std::vector<double>& cent, int leaf_threshold,

@@ -83,14 +83,16 @@ class GpuScene:
         self.patch_begin = np.array([0, sc.n_patches], np.int64)
 
     @classmethod
-    def batch(cls, scenes, compact: bool = False):
-        """Several independent scenes in one handle (bbrt_scene_create_ex; compact: coarse cells on the fly)."""
+    def batch(cls, scenes, compact: bool = False, device_build: bool = False):
+        """Several independent scenes in one handle (bbrt_scene_create_ex; compact: coarse cells on the fly;
+        device_build: the compact scene's per-cell build work on the GPU)."""
         self = cls(None)
         sets = (L.PatchSet * len(scenes))(*[L.patch_set(sc) for sc in scenes])
         self._keep = scenes
         h = C.c_void_p()
-        L.check(L.lib().bbrt_scene_create_ex(sets, C.c_int32(len(scenes)), C.c_uint32(L.SCENE_COMPACT if compact else 0),
-                                             C.byref(h)), "bbrt_scene_create_ex")
+        fl = (L.SCENE_COMPACT if compact else 0) | (L.SCENE_DEVICE_BUILD if device_build else 0)
+        L.check(L.lib().bbrt_scene_create_ex(sets, C.c_int32(len(scenes)), C.c_uint32(fl), C.byref(h)),
+                "bbrt_scene_create_ex")
         self.h = h
         self.n_scenes = len(scenes)
         self.patch_begin = np.zeros(self.n_scenes + 1, np.int64)

