@@ -9,7 +9,9 @@ ray batch, inputs and outputs resident in HBM.
 Prints one JSON line with the throughput, the FP64 roofline of the bounce
 kernel (algorithmic Newton flops / summed kernel time, HIP events on the
 stream the kernels run on) and the CPU oracle timed on a bounded sample of the
-same rays on the host cores.
+same rays on the host cores.  Two more legs ride along in the same line: the
+learned Newton seed (trained on the GPU first, then the same trace) and the
+batched bounce map (256 independent scenes in one hipGraph-replayed launch).
 """
 import argparse
 import json
@@ -46,6 +48,9 @@ def main():
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--train-lr", type=float, default=3e-3)
     ap.add_argument("--newton-batch", type=int, default=0, help="lanes that run Newton together in the persistent kernel (0 = library default)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -154,6 +159,11 @@ def main():
             res["learned"] = learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value, res)
             res["mean_newton_iters"]["learned"] = res["learned"]["mean_newton_iters"]
 
+    if not args.no_batch:
+        del to, td, outs
+        torch.cuda.empty_cache()
+        res["batch"] = batch_leg(args)
+
     traffic = committed_traffic(args, n, mb, K)
     if traffic is not None:
         res["roofline"]["traffic"] = traffic
@@ -161,6 +171,69 @@ def main():
     if args.cpu_seconds > 0:
         res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, mode, args.cpu_seconds, gs, K)
     print(json.dumps(res), flush=True)
+
+
+def batch_leg(args):
+    """The batched bounce map: args.batch_scenes independent scenes (1-8 obstacles, patches log-uniform in
+    [80, 5120] per obstacle, order 2/4, 1k-64k rays each, 30 degree cone, 50 bounces) in one compact scene
+    handle built on the device, launch shape autotuned, traced by one hipGraph-replayed persistent launch."""
+    import numpy as np
+    import torch
+    from billardballraytracing3d_amd import scenes as S, tracer as T, _lib as L
+
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    scenes, o, d, rb = S.batch_config(args.seed, n_scenes=args.batch_scenes)
+    t1 = time.time()
+    gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
+    t2 = time.time()
+    n_patches = gb.n_patches
+    del scenes
+    n = int(rb[-1])
+    mb, K = 50, 20
+    to = torch.from_numpy(o).to(dev)
+    td = torch.from_numpy(d).to(dev)
+    outs = dict(status=torch.empty(n, dtype=torch.int8, device=dev),
+                err_kind=torch.empty(n, dtype=torch.int8, device=dev),
+                bounces=torch.empty(n, dtype=torch.int32, device=dev),
+                hit_patch=torch.empty((n, K), dtype=torch.int32, device=dev),
+                hit_xyz=torch.empty((n, K, 3), dtype=torch.float64, device=dev))
+    ptrs = {k: v.data_ptr() for k, v in outs.items()}
+    st = torch.cuda.current_stream(dev).cuda_stream
+    ta = time.time()
+    best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st)
+    t_tune = time.time() - ta
+
+    def step():
+        return gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
+                               flags=L.FLAG_GRAPH, **best)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    ctr = np.zeros(L.NCOUNTERS, np.int64)
+    kms = np.zeros(3)
+    ts = time.perf_counter()
+    for _ in range(args.batch_steps):
+        c, k = step()
+        ctr += c
+        kms += k
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    bounces = int(ctr[L.CTR["bounces"]])
+    flops = float(ctr[L.CTR["newton_flops"]]) + 27.0 * bounces
+    graph_ms = kms[0] / args.batch_steps
+    return {
+        "workload": f"batched-bounce-map-{args.batch_scenes}-scenes", "value": bounces / (te - ts),
+        "unit": "bounces/s", "ms_per_step": (te - ts) * 1e3 / args.batch_steps, "steps": args.batch_steps,
+        "scenes": args.batch_scenes, "patches": n_patches, "rays": n, "max_bounces": mb, "record_k": K,
+        "newton_mode": "converged", "scene": "compact (coarse cells evaluated on the fly), device build",
+        "bounces_per_step": bounces // args.batch_steps, "graph_ms_per_step": graph_ms,
+        "fp64_tflops": flops / (graph_ms * args.batch_steps * 1e-3) / 1e12 if graph_ms > 0 else None,
+        "autotuned": best, "autotune_candidates": rows, "autotune_s": t_tune,
+        "scene_generation_s": t1 - t0, "scene_build_s": t2 - t1,
+        "counters_per_step": {k: int(ctr[v] // args.batch_steps) for k, v in L.CTR.items()},
+    }
 
 
 MLP_FLOPS_PER_WAVE_EVAL = 160 * 16 * 16 * 16 * 2   # 160 bf16 16x16x16 matrix products per 64-lane evaluation

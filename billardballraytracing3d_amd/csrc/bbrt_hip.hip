@@ -115,7 +115,7 @@ enum : int { kPhTrav = 1, kPhLeaf = 2, kPhDone = 3, kPhSeed = 4 };
 // kept only under the acceptance rule (converged, polished, within
 // mlp::kSeedWindow of the triangle seed), else Newton runs again from the
 // triangle seed and its outcome stands.
-template <int W, bool WORK, bool LEARNED>
+template <int W, bool WORK, bool LEARNED, bool COMPACT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_t* __restrict__ n_live_p,
           int32_t* head, Outs O, int max_bounces, int record_k, int mode, uint32_t flags, unsigned long long* ctr,
@@ -240,7 +240,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
             obj = S.leaf[leaf_start + q];
             have_h = true;
             if (enter & (1u << q)) {
-                sd = stage1_seed(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
+                sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
                 if (sd.have) { phase = kPhSeed; have_h = false; }   // parked until the wave runs Newton
             } else {
                 c.candidates++;
@@ -552,27 +552,29 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     const bool graph = (p->flags & BBRT_FLAG_GRAPH) != 0;
     if (graph && wavefront) throw std::runtime_error("bbrt_trace: BBRT_FLAG_GRAPH needs the persistent schedule");
     // the persistent kernel and its grid (host-side choices, fixed in a captured graph)
-    decltype(&k_persist<2, false, false>) kern = nullptr;
+    decltype(&k_persist<2, false, false, false>) kern = nullptr;
     unsigned grid = 1;
     const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
     const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch;
     if (!wavefront && p->max_bounces > 0) {
         const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : kDefaultWavesPerEu;
         const bool work = O.ray_work != nullptr;
-        if (learned) {
-            switch (wpe) {
-                case 2: kern = &k_persist<2, false, true>; break;
-                case 3: kern = &k_persist<3, false, true>; break;
-                case 4: kern = &k_persist<4, false, true>; break;
-                default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
-            }
+        if (wpe != 2 && wpe != 3 && wpe != 4) throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
+        if (s->compact && work) throw std::runtime_error("bbrt_trace: ray_work is not recorded on a compact scene");
+        if (s->compact) {
+            if (learned) kern = wpe == 2 ? &k_persist<2, false, true, true> : (wpe == 3 ? &k_persist<3, false, true, true>
+                                                                                        : &k_persist<4, false, true, true>);
+            else kern = wpe == 2 ? &k_persist<2, false, false, true> : (wpe == 3 ? &k_persist<3, false, false, true>
+                                                                                  : &k_persist<4, false, false, true>);
+        } else if (learned) {
+            kern = wpe == 2 ? &k_persist<2, false, true, false> : (wpe == 3 ? &k_persist<3, false, true, false>
+                                                                             : &k_persist<4, false, true, false>);
+        } else if (work) {
+            kern = wpe == 2 ? &k_persist<2, true, false, false> : (wpe == 3 ? &k_persist<3, true, false, false>
+                                                                             : &k_persist<4, true, false, false>);
         } else {
-            switch (wpe) {
-                case 2: kern = work ? &k_persist<2, true, false> : &k_persist<2, false, false>; break;
-                case 3: kern = work ? &k_persist<3, true, false> : &k_persist<3, false, false>; break;
-                case 4: kern = work ? &k_persist<4, true, false> : &k_persist<4, false, false>; break;
-                default: throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
-            }
+            kern = wpe == 2 ? &k_persist<2, false, false, false> : (wpe == 3 ? &k_persist<3, false, false, false>
+                                                                              : &k_persist<4, false, false, false>);
         }
         int per_cu = 0;
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0), "occupancy");

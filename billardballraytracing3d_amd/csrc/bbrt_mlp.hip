@@ -192,7 +192,7 @@ __global__ void k_mlp_gen(DevScene S, const double* frame, const float* pfeat, i
     if (g >= n) return;
     const Pair pr = draw_pair(S, frame, n_patch, seed, first + g);
     Counters c = {};
-    const Seed sd = stage1_seed(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, S.scenes[0].cull_widen, c);
+    const Seed sd = stage1_seed<2>(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, S.scenes[0].cull_widen, c);
     PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     if (sd.have) h = newton_any(S, pr.patch, pr.o, pr.d, sd.u, sd.v, sd.t, BBRT_NEWTON_CONVERGED, c);
     const bool ok = h.status == 1;
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256) k_mlp_ab(DevScene S, MlpDev M, int64_t n_
     PHit ht{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     if (in) {
         pr = draw_pair(S, M.frame, n_patch, seed, first + g);
-        sd = stage1_seed(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, S.scenes[0].cull_widen, c);
+        sd = stage1_seed<2>(S, pr.patch, pr.o, pr.d, ray32_of(pr.o, pr.d), true, S.scenes[0].cull_widen, c);
         if (sd.have) ht = newton_any(S, pr.patch, pr.o, pr.d, sd.u, sd.v, sd.t, BBRT_NEWTON_CONVERGED, c);
     }
     double ul, vl, tl;

@@ -383,6 +383,8 @@ __device__ __forceinline__ uint32_t group_mask(const float* __restrict__ g, cons
 }
 
 // Stage 1 through the culling pyramid (grouped layout: one load per level).
+// CM: 0 stored scenes only, 1 compact scenes only, 2 either (S.compact decides at run time).
+template <int CM>
 __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, const double* C, const V3& o,
                               const V3& d, const Ray32& r, double widen, Stage1& best, Counters& c) {
     const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
@@ -407,7 +409,7 @@ __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, con
                 const int tq = __ffs(mt) - 1;
                 mt &= mt - 1;
                 const int ti = 3 * bi + tq / 3, tj = 3 * bj + tq % 3;
-                if (S.compact) {
+                if (CM == 1 || (CM == 2 && S.compact)) {
                     // the quantized boxes (supersets) select cells; those are evaluated from the
                     // fields, their exact box rebuilt and tested as stored, then Moller-Trumbore
                     const float* tb = gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9 + 6 * tq;
@@ -639,14 +641,15 @@ struct Seed {
     bool have;
 };
 
+template <int CM>
 __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32,
                                             bool cull, double widen, Counters& c) {
     const double* G = S.cxyz + (size_t)k * bbrt::kCoarseN * bbrt::kCoarseN * 3;
     const double* C = S.cell + (size_t)k * bbrt::kCells * bbrt::kCells * 4;
     Stage1 best{ 0.0, 0, false };
     c.candidates++;
-    if (cull && S.meta[k].cull) stage1_culled(S, k, G, C, o, d, r32, widen, best, c);
-    else if (S.compact) stage1_brute_compact(S, k, o, d, best, c);
+    if (cull && S.meta[k].cull) stage1_culled<CM>(S, k, G, C, o, d, r32, widen, best, c);
+    else if (CM == 1 || (CM == 2 && S.compact)) stage1_brute_compact(S, k, o, d, best, c);
     else stage1_brute(G, C, o, d, best, c);
     Seed sd{ 0.0, 0.0, best.t, best.have };
     if (!best.have) return sd;
@@ -674,7 +677,7 @@ __device__ __forceinline__ PHit newton_any(const DevScene& S, int64_t k, const V
 // SurfacePatch::intersect (SurfacePatch.hpp:253-303): stage 1, then Newton from its seed.
 __device__ PHit patch_intersect(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32, bool cull,
                                 double widen, int mode, Counters& c) {
-    const Seed sd = stage1_seed(S, k, o, d, r32, cull, widen, c);
+    const Seed sd = stage1_seed<2>(S, k, o, d, r32, cull, widen, c);
     if (!sd.have) return PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     return newton_any(S, k, o, d, sd.u, sd.v, sd.t, mode, c);
 }
