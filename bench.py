@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-no-tune", action="store_true", help="batched leg: library default launch shape")
+    ap.add_argument("--batch-only", action="store_true", help="run the batched-scenes leg alone")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -70,6 +72,13 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
+    if args.batch_only:
+        b = batch_leg(args)
+        print(json.dumps({"metric": "converged bounces/s", "value": b["value"], "unit": "bounces/s", "n_gpus": 1,
+                          "steps": b["steps"], "warmup": args.warmup, "ms_per_step": b["ms_per_step"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic", "config": {"workload": b["workload"]}, "batch": b}), flush=True)
+        return
     t0 = time.time()
     sc, o, d, mb = S.config2(seed=args.seed, n_rays=args.rays)
     mb = args.max_bounces
@@ -201,7 +210,10 @@ def batch_leg(args):
     ptrs = {k: v.data_ptr() for k, v in outs.items()}
     st = torch.cuda.current_stream(dev).cuda_stream
     ta = time.time()
-    best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st)
+    if args.batch_no_tune:
+        best, rows = {}, []
+    else:
+        best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st)
     t_tune = time.time() - ta
 
     def step():

@@ -46,7 +46,7 @@ namespace {
 // normalize_dir (:398-401); then the root box test of the first BVH query
 // (BVH.hpp:91-97 with best_t = inf), so rays that miss the whole scene leave
 // here as Escaped after one node test.  The rest enter the live list.
-__global__ void k_init(const double* __restrict__ origins, const double* __restrict__ dirs, int64_t n,
+__global__ void __launch_bounds__(1024) k_init(const double* __restrict__ origins, const double* __restrict__ dirs, int64_t n,
                        DevScene S, RayState R, Outs O, int max_bounces, int32_t* live_out,
                        int32_t* n_out, unsigned long long* ctr) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,12 +87,31 @@ __global__ void k_init(const double* __restrict__ origins, const double* __restr
             }
         }
     }
-    append_live(alive, r, live_out, n_out);
+    // One atomic per 1024-thread block for the live list and each counter:
+    // per-wave atomics on one address serialise (16k of them for 2^20 rays).
+    __shared__ int w_live[16];
+    __shared__ unsigned long long w_rays[16], w_tests[16];
+    __shared__ int block_base;
+    const unsigned long long mask = __ballot(alive);
     const unsigned long long in_range = __ballot(g < n);
     const unsigned long long rt = wave_sum(root_tests);
-    if (__lane_id() == 0) {
-        if (in_range) atomicAdd(ctr + bbrt::kCtrRays, (unsigned long long)__popcll(in_range));
-        if (rt) atomicAdd(ctr + bbrt::kCtrBvh, rt);
+    const int wid = threadIdx.x >> 6, lane = __lane_id();
+    if (lane == 0) { w_live[wid] = (int)__popcll(mask); w_rays[wid] = __popcll(in_range); w_tests[wid] = rt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        int tot = 0;
+        unsigned long long rays = 0, tests = 0;
+        for (int w = 0; w < nw; ++w) { tot += w_live[w]; rays += w_rays[w]; tests += w_tests[w]; }
+        block_base = tot ? atomicAdd(n_out, tot) : 0;
+        if (rays) atomicAdd(ctr + bbrt::kCtrRays, rays);
+        if (tests) atomicAdd(ctr + bbrt::kCtrBvh, tests);
+    }
+    __syncthreads();
+    if (alive) {
+        int off = block_base;
+        for (int w = 0; w < wid; ++w) off += w_live[w];
+        live_out[off + (int)__popcll(mask & ((1ull << lane) - 1ull))] = r;
     }
 }
 
@@ -583,7 +602,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             resident = std::max<int64_t>(1, (int64_t)((double)resident * p->grid_fraction));
         grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, resident));
     }
-    const int64_t grid0 = (n + 255) / 256;
+    const int64_t grid0 = (n + 1023) / 1024;
     // zero the counters, optional record padding, k_init, the persistent kernel
     auto enqueue = [&](hipStream_t q, bool events) {
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, q, cnt, 4);
@@ -593,7 +612,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             const unsigned pg = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
             hipLaunchKernelGGL(k_pad_records, dim3(pg), dim3(256), 0, q, O, total);
         }
-        hipLaunchKernelGGL(k_init, dim3((unsigned)grid0), dim3(256), 0, q, d_o, d_d, n, dev, R, O,
+        hipLaunchKernelGGL(k_init, dim3((unsigned)grid0), dim3(1024), 0, q, d_o, d_d, n, dev, R, O,
                            p->max_bounces, live[0], cnt, ctr);
         if (kern) {
             if (events) check(hipEventRecord(s->kev[0], q), "hipEventRecord");
