@@ -53,6 +53,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-no-tune", action="store_true", help="batched leg: library default launch shape")
     ap.add_argument("--batch-only", action="store_true", help="run the batched-scenes leg alone")
+    ap.add_argument("--workload", choices=["sinai", "c5"], default="sinai",
+                    help="c5: the full batched form (512 scenes, 16M rays, 200 bounces, learned seeds) alone")
+    ap.add_argument("--c5-scenes", type=int, default=512)
+    ap.add_argument("--c5-rays", type=int, default=16_000_000)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -72,6 +76,10 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
+    if args.workload == "c5":
+        r = c5_leg(args)
+        print(json.dumps(r), flush=True)
+        return
     if args.batch_only:
         b = batch_leg(args)
         print(json.dumps({"metric": "converged bounces/s", "value": b["value"], "unit": "bounces/s", "n_gpus": 1,
@@ -245,6 +253,109 @@ def batch_leg(args):
         "autotuned": best, "autotune_candidates": rows, "autotune_s": t_tune,
         "scene_generation_s": t1 - t0, "scene_build_s": t2 - t1,
         "counters_per_step": {k: int(ctr[v] // args.batch_steps) for k, v in L.CTR.items()},
+    }
+
+
+def c5_leg(args):
+    """The full batched form: args.c5_scenes scenes of the batched layout, args.c5_rays rays in total, 200
+    bounces, learned Newton seeds (network trained on the GPU on pairs of the Sinai scene) with the
+    triangle-seed fallback, one hipGraph-replayed persistent launch with an autotuned shape.  Reports the
+    fallback rate per scene and checks sampled scenes against the CPU oracle (triangle seed, converged)."""
+    import numpy as np
+    import torch
+    from billardballraytracing3d_amd import scenes as S, tracer as T, _lib as L
+    from oracle import oracle as O
+
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    scenes, o, d, rb = S.batch_config(args.seed, n_scenes=args.c5_scenes, total_rays=args.c5_rays)
+    t1 = time.time()
+    gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
+    t2 = time.time()
+    # the network: trained on the Sinai scene (a compact handle of its own), then attached to the batch
+    sinai = T.GpuScene.batch([S.sinai_scene(0)], compact=True, device_build=True)
+    net = T.Mlp(args.seed + 3)
+    rep, _ = net.train(sinai, n_samples=args.train_samples, seed=21 + args.seed, batch=args.train_batch,
+                       epochs=args.train_epochs, lr=args.train_lr, lr_final=args.train_lr / 100)
+    del sinai
+    gb.set_mlp(net)
+    t3 = time.time()
+    n = int(rb[-1])
+    mb, K = 200, 8
+    to = torch.from_numpy(o).to(dev)
+    td = torch.from_numpy(d).to(dev)
+    outs = dict(status=torch.empty(n, dtype=torch.int8, device=dev),
+                err_kind=torch.empty(n, dtype=torch.int8, device=dev),
+                bounces=torch.empty(n, dtype=torch.int32, device=dev),
+                hit_patch=torch.empty((n, K), dtype=torch.int32, device=dev),
+                hit_xyz=torch.empty((n, K, 3), dtype=torch.float64, device=dev),
+                seed_stats=torch.empty((n, 2), dtype=torch.int32, device=dev))
+    ptrs = {k: v.data_ptr() for k, v in outs.items()}
+    st = torch.cuda.current_stream(dev).cuda_stream
+    best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
+                             seed=L.SEED_LEARNED)
+
+    def step():
+        return gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
+                               flags=L.FLAG_GRAPH, seed=L.SEED_LEARNED, **best)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 5))
+    ctr = np.zeros(L.NCOUNTERS, np.int64)
+    ts = time.perf_counter()
+    for _ in range(steps):
+        c, _ = step()
+        ctr += c
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    bounces = int(ctr[L.CTR["bounces"]])
+    # fallback rate per scene
+    ss = outs["seed_stats"].cpu().numpy().astype(np.int64)
+    per = []
+    for i in range(len(scenes)):
+        a = ss[rb[i]:rb[i + 1]].sum(axis=0)
+        if a.sum() > 0:
+            per.append(a[1] / a.sum())
+    per = np.array(per)
+    # oracle check on sampled scenes: same rays, triangle seed, converged Newton, FP64
+    status = outs["status"].cpu().numpy(); bnc = outs["bounces"].cpu().numpy()
+    hp = outs["hit_patch"].cpu().numpy(); hx = outs["hit_xyz"].cpu().numpy()
+    rays_checked = mism = 0
+    max_err = 0.0
+    threads = min(os.cpu_count() or 1, 16)
+    for i in np.linspace(0, len(scenes) - 1, 6).astype(int):
+        sl = slice(int(rb[i]), int(min(rb[i + 1], rb[i] + 1500)))
+        r = O.OracleScene(scenes[i], threads=threads).trace(o[sl], d[sl], mb, mode=L.NEWTON_CONVERGED, record_k=K,
+                                                            threads=threads)
+        kk = np.minimum(r.bounces, K)
+        m = np.arange(K)[None, :] < kk[:, None]
+        bad = (status[sl] != r.status) | (bnc[sl] != r.bounces) | np.any(np.where(m, hp[sl] != r.hit_patch, False), 1)
+        mism += int(bad.sum())
+        rays_checked += r.status.size
+        scale = np.maximum(np.linalg.norm(r.hit_xyz, axis=-1), 1.0)
+        err = np.where(m, np.linalg.norm(hx[sl] - r.hit_xyz, axis=-1) / scale, 0.0)
+        max_err = max(max_err, float(err.max()))
+    hits = int(ctr[L.CTR["seed_learned"]] + ctr[L.CTR["seed_fallback"]])
+    return {
+        "metric": METRIC, "value": bounces / (te - ts), "unit": "bounces/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, args.warmup), "ms_per_step": (te - ts) * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched-full-form-{len(scenes)}-scenes-{n}-rays", "scenes": len(scenes),
+                   "patches": gb.n_patches, "rays": n, "max_bounces": mb, "record_k": K, "newton_mode": "converged",
+                   "seed_mode": "learned", "scene": "compact, device build", "autotuned": best},
+        "mean_newton_iters": {"learned": int(ctr[L.CTR["newton_iters"]]) / hits if hits else None},
+        "seed_learned_per_step": int(ctr[L.CTR["seed_learned"]]) // steps,
+        "seed_fallback_per_step": int(ctr[L.CTR["seed_fallback"]]) // steps,
+        "fallback_rate_per_scene": {"scenes": int(per.size), "min": float(per.min()), "median": float(np.median(per)),
+                                    "mean": float(per.mean()), "max": float(per.max())} if per.size else None,
+        "oracle_check": {"scenes": 6, "rays": rays_checked, "rays_with_mismatch": mism, "max_rel_hit_error": max_err,
+                         "oracle": "CPU FP64, triangle seed, converged Newton"},
+        "bounces_per_step": bounces // steps, "train": {"pairs": rep["n_pairs"], "steps": rep["steps"],
+                                                         "test_loss": rep["test_loss"]},
+        "scene_generation_s": t1 - t0, "scene_build_s": t2 - t1, "train_s": t3 - t2,
+        "counters_per_step": {k: int(ctr[v] // steps) for k, v in L.CTR.items()},
     }
 
 

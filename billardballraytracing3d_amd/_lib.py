@@ -62,7 +62,8 @@ class TuneEntry(C.Structure):
 class Outputs(C.Structure):
     _fields_ = [("status", C.c_void_p), ("err_kind", C.c_void_p), ("bounces", C.c_void_p), ("err_patch", C.c_void_p),
                 ("hit_patch", C.c_void_p), ("hit_xyz", C.c_void_p), ("newton_iters", C.c_void_p),
-                ("final_od", C.c_void_p), ("hit_count", C.c_void_p), ("ray_work", C.c_void_p)]
+                ("final_od", C.c_void_p), ("hit_count", C.c_void_p), ("ray_work", C.c_void_p),
+                ("seed_stats", C.c_void_p)]
 
 
 class TrainConfig(C.Structure):

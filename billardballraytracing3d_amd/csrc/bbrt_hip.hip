@@ -57,6 +57,7 @@ __global__ void __launch_bounds__(1024) k_init(const double* __restrict__ origin
         const V3 o = { origins[3 * g], origins[3 * g + 1], origins[3 * g + 2] };
         const V3 dr = { dirs[3 * g], dirs[3 * g + 1], dirs[3 * g + 2] };
         if (O.ray_work) { O.ray_work[4 * g] = 0; O.ray_work[4 * g + 1] = 0; O.ray_work[4 * g + 2] = 0; O.ray_work[4 * g + 3] = 0; }
+        if (O.seed_stats) { O.seed_stats[2 * g] = 0; O.seed_stats[2 * g + 1] = 0; }
         if (O.err_kind) O.err_kind[g] = bbrt::kErrNone;
         if (O.err_patch) O.err_patch[g] = -1;
         const double len2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
@@ -167,6 +168,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
     int leaf_start = 0, leaf_count = 0, q = 0;
     uint32_t enter = 0;
     int32_t obj = 0;                          // the leaf object being intersected
+    uint32_t sl0 = 0, sf0 = 0;                // learned mode: seed counters when the ray started
     Seed sd{ 0.0, 0.0, 0.0, false };          // its stage-1 seed (learned mode, while parked)
 
     // bounce set-up: RayTracingDriver.hpp:187 and the root of BVH.hpp:89
@@ -197,6 +199,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 if (qq < n_live) {
                     r = live[qq];
                     scn = ray_scene_of(S, r);
+                    if constexpr (LEARNED) { sl0 = c.seed_learned; sf0 = c.seed_fallback; }
                     o = { R.o[3 * r], R.o[3 * r + 1], R.o[3 * r + 2] };
                     d = { R.d[3 * r], R.d[3 * r + 1], R.d[3 * r + 2] };
                     b = 0;
@@ -305,6 +308,11 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
             const bool alive = finish_bounce(S, O, r, scn, o, d, b, max_bounces, record_k, flags, err, err_obj, best,
                                              best_obj, c);
             if constexpr (WORK) add_ray_work(O, r, c, c0);
+            if constexpr (LEARNED)
+                if (!alive && O.seed_stats) {
+                    O.seed_stats[2 * (size_t)r] = c.seed_learned - sl0;
+                    O.seed_stats[2 * (size_t)r + 1] = c.seed_fallback - sf0;
+                }
             if (alive) { ++b; start_bounce(false); }
             else { r = -1; phase = 0; }
         }
@@ -557,7 +565,8 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     Outs O{ out ? out->status : nullptr, out ? out->err_kind : nullptr, out ? out->bounces : nullptr,
             out ? out->err_patch : nullptr, out ? out->hit_patch : nullptr, out ? out->hit_xyz : nullptr,
             out ? out->newton_iters : nullptr, out ? out->final_od : nullptr,
-            out ? reinterpret_cast<unsigned long long*>(out->hit_count) : nullptr, out ? out->ray_work : nullptr };
+            out ? reinterpret_cast<unsigned long long*>(out->hit_count) : nullptr, out ? out->ray_work : nullptr,
+            out ? out->seed_stats : nullptr };
     const bool timing = (p->flags & BBRT_FLAG_TIME_KERNELS) != 0;
     const bool wavefront = (p->flags & BBRT_FLAG_WAVEFRONT) != 0;
     if (timing && (int)s->kev.size() < 2 * p->max_bounces + 2) {
@@ -1240,6 +1249,7 @@ int trace_host(bbrt_scene s, const double* origins, const double* dirs, int64_t 
         if (hout.newton_iters) dout.newton_iters = static_cast<uint8_t*>(dalloc(nn * K));
         if (hout.final_od) dout.final_od = static_cast<double*>(dalloc(nn * 48));
         if (hout.ray_work) dout.ray_work = static_cast<uint32_t*>(dalloc(nn * 16));
+        if (hout.seed_stats) dout.seed_stats = static_cast<uint32_t*>(dalloc(nn * 8));
         if (hout.hit_count) {
             dout.hit_count = static_cast<int64_t*>(dalloc((size_t)s->n_patches * 8));
             check(hipMemcpy(dout.hit_count, hout.hit_count, (size_t)s->n_patches * 8, hipMemcpyHostToDevice), "H2D hit_count");
@@ -1258,6 +1268,7 @@ int trace_host(bbrt_scene s, const double* origins, const double* dirs, int64_t 
         back(hout.final_od, dout.final_od, nn * 48);
         back(hout.hit_count, dout.hit_count, (size_t)s->n_patches * 8);
         back(hout.ray_work, dout.ray_work, nn * 16);
+        back(hout.seed_stats, dout.seed_stats, nn * 8);
         cleanup();
         return 0;
     } catch (const std::exception& e) {

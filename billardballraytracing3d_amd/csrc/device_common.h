@@ -68,6 +68,7 @@ struct Outs {
     double* final_od;
     unsigned long long* hit_count;
     uint32_t* ray_work;
+    uint32_t* seed_stats;
 };
 
 struct V3 { double x, y, z; };

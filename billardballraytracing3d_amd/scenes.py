@@ -462,18 +462,28 @@ def batch_obstacles(rng: "SplitMix64", patch_lo: float, patch_hi: float) -> List
 
 
 def batch_config(seed: int = 0, n_scenes: int = 256, patch_lo: float = 80, patch_hi: float = 5120,
-                 rays_lo: float = 1000, rays_hi: float = 64000, cone_deg: float = 30.0):
+                 rays_lo: float = 1000, rays_hi: float = 64000, cone_deg: float = 30.0,
+                 total_rays: Optional[int] = None):
     """The batched bounce map: n_scenes independent scenes (batch_obstacles) with ray counts log-uniform in
     [rays_lo, rays_hi], origins uniform on z = -8 over [-6, 6]^2, directions uniform in a cone about +z.
+    With total_rays the drawn counts are scaled (keeping their proportions) to sum to it.
 
     Returns (scenes, origins, dirs, ray_begin): the rays of scene i are ray_begin[i] .. ray_begin[i+1]-1.
     """
     rng = SplitMix64(0xBA7C4 + seed)
-    scenes, os_, ds_ = [], [], []
-    begin = [0]
+    scenes, counts = [], []
     for k in range(n_scenes):
         scenes.append(build_scene(batch_obstacles(rng, patch_lo, patch_hi)))
-        nr = int(round(math.exp(rng.uniform(1, math.log(rays_lo), math.log(rays_hi))[0])))
+        counts.append(math.exp(rng.uniform(1, math.log(rays_lo), math.log(rays_hi))[0]))
+    if total_rays is not None:
+        scale = total_rays / sum(counts)
+        counts = [c * scale for c in counts]
+    counts = [max(1, int(round(c))) for c in counts]
+    if total_rays is not None:
+        counts[-1] = max(1, counts[-1] + total_rays - sum(counts))
+    os_, ds_ = [], []
+    begin = [0]
+    for k, nr in enumerate(counts):
         o, d = cone_rays(nr, (seed << 16) + k, 6.0, -8.0, cone_deg)
         os_.append(o); ds_.append(d)
         begin.append(begin[-1] + nr)

@@ -28,6 +28,7 @@ class TraceResult:
     counters: np.ndarray     # int64 [16]
     hit_count: Optional[np.ndarray] = None
     ray_work: Optional[np.ndarray] = None   # uint32 [n, 4]
+    seed_stats: Optional[np.ndarray] = None # uint32 [n, 2]: learned roots kept / fallbacks
 
     def counter(self, name: str) -> int:
         return int(self.counters[L.CTR[name]])
@@ -66,7 +67,7 @@ def _params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newto
 def _outputs(out_ptrs: dict):
     return L.Outputs(*[C.c_void_p(out_ptrs.get(k) or 0) if out_ptrs.get(k) else None
                        for k in ("status", "err_kind", "bounces", "err_patch", "hit_patch", "hit_xyz",
-                                 "newton_iters", "final_od", "hit_count", "ray_work")])
+                                 "newton_iters", "final_od", "hit_count", "ray_work", "seed_stats")])
 
 
 class GpuScene:
@@ -144,7 +145,8 @@ class GpuScene:
     def trace(self, origins, dirs, max_bounces: int, mode: int = L.NEWTON_CONVERGED,
               record_k: Optional[int] = None, flags: int = 0, hit_count: bool = False, block: int = 0,
               ray_work: bool = False, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE,
-              newton_batch: int = 0, grid_fraction: float = 0.0, ray_begin=None) -> TraceResult:
+              newton_batch: int = 0, grid_fraction: float = 0.0, ray_begin=None,
+              seed_stats: bool = False) -> TraceResult:
         o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
         d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
         if o.shape != d.shape:
@@ -155,11 +157,12 @@ class GpuScene:
                           np.zeros((n, K), np.int32), np.zeros((n, K, 3)), np.zeros((n, K), np.uint8),
                           np.zeros((n, 6)), np.zeros(L.NCOUNTERS, np.int64),
                           np.zeros(self.n_patches, np.int64) if hit_count else None,
-                          np.zeros((n, 4), np.uint32) if ray_work else None)
+                          np.zeros((n, 4), np.uint32) if ray_work else None,
+                          np.zeros((n, 2), np.uint32) if seed_stats else None)
         p = _params(mode, seed, max_bounces, K, flags, block, waves_per_eu, newton_batch, grid_fraction)
         out = L.Outputs(L.vptr(res.status), L.vptr(res.err_kind), L.vptr(res.bounces), L.vptr(res.err_patch),
                         L.vptr(res.hit_patch), L.vptr(res.hit_xyz), L.vptr(res.iters), L.vptr(res.final_od),
-                        L.vptr(res.hit_count), L.vptr(res.ray_work))
+                        L.vptr(res.hit_count), L.vptr(res.ray_work), L.vptr(res.seed_stats))
         if ray_begin is None:
             rc = L.lib().bbrt_trace(self.h, L.ptr(o, L._d), L.ptr(d, L._d), C.c_int64(n), C.byref(p), C.byref(out),
                                     L.ptr(res.counters, L._i64))

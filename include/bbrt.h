@@ -112,6 +112,8 @@ typedef struct {
     int64_t* hit_count;
     uint32_t* ray_work;   /* [n][4]: BVH node tests, culling box tests, Moller-Trumbore tests and
                              Newton iterations spent on each ray, summed over its bounces */
+    uint32_t* seed_stats; /* [n][2], learned-seed mode: the ray's stage-1 hits whose learned-seed root was
+                             kept, and those that fell back to the triangle seed (zeros otherwise) */
 } bbrt_outputs;
 
 /* Work counters returned by the trace calls (int64[BBRT_NCOUNTERS]). */
