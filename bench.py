@@ -256,6 +256,10 @@ def batch_leg(args):
     }
 
 
+def _progress(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def c5_leg(args):
     """The full batched form: args.c5_scenes scenes of the batched layout, args.c5_rays rays in total, 200
     bounces, learned Newton seeds (network trained on the GPU on pairs of the Sinai scene) with the
@@ -270,8 +274,10 @@ def c5_leg(args):
     t0 = time.time()
     scenes, o, d, rb = S.batch_config(args.seed, n_scenes=args.c5_scenes, total_rays=args.c5_rays)
     t1 = time.time()
+    _progress(f"c5: {len(scenes)} scenes, {int(rb[-1])} rays generated in {t1 - t0:.1f} s")
     gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
     t2 = time.time()
+    _progress(f"c5: {gb.n_patches} patches built (compact, device build) in {t2 - t1:.1f} s")
     # the network: trained on the Sinai scene (a compact handle of its own), then attached to the batch
     sinai = T.GpuScene.batch([S.sinai_scene(0)], compact=True, device_build=True)
     net = T.Mlp(args.seed + 3)
@@ -280,6 +286,7 @@ def c5_leg(args):
     del sinai
     gb.set_mlp(net)
     t3 = time.time()
+    _progress(f"c5: network trained in {t3 - t2:.1f} s, test loss {rep['test_loss']:.3g}")
     n = int(rb[-1])
     mb, K = 200, 8
     to = torch.from_numpy(o).to(dev)
@@ -294,6 +301,7 @@ def c5_leg(args):
     st = torch.cuda.current_stream(dev).cuda_stream
     best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
                              seed=L.SEED_LEARNED)
+    _progress(f"c5: autotuned {best} over {len(rows)} shapes")
 
     def step():
         return gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
@@ -311,6 +319,7 @@ def c5_leg(args):
     torch.cuda.synchronize()
     te = time.perf_counter()
     bounces = int(ctr[L.CTR["bounces"]])
+    _progress(f"c5: {steps} steps, {(te - ts) * 1e3 / steps:.1f} ms/step, {bounces / (te - ts):.4g} bounces/s")
     # fallback rate per scene
     ss = outs["seed_stats"].cpu().numpy().astype(np.int64)
     per = []
@@ -326,7 +335,7 @@ def c5_leg(args):
     max_err = 0.0
     threads = min(os.cpu_count() or 1, 16)
     for i in np.linspace(0, len(scenes) - 1, 6).astype(int):
-        sl = slice(int(rb[i]), int(min(rb[i + 1], rb[i] + 1500)))
+        sl = slice(int(rb[i]), int(min(rb[i + 1], rb[i] + 600)))
         r = O.OracleScene(scenes[i], threads=threads).trace(o[sl], d[sl], mb, mode=L.NEWTON_CONVERGED, record_k=K,
                                                             threads=threads)
         kk = np.minimum(r.bounces, K)
@@ -337,6 +346,7 @@ def c5_leg(args):
         scale = np.maximum(np.linalg.norm(r.hit_xyz, axis=-1), 1.0)
         err = np.where(m, np.linalg.norm(hx[sl] - r.hit_xyz, axis=-1) / scale, 0.0)
         max_err = max(max_err, float(err.max()))
+        _progress(f"c5: oracle check scene {i}: {r.status.size} rays, {mism} mismatches so far")
     hits = int(ctr[L.CTR["seed_learned"]] + ctr[L.CTR["seed_fallback"]])
     return {
         "metric": METRIC, "value": bounces / (te - ts), "unit": "bounces/s", "n_gpus": 1, "steps": steps,

@@ -585,7 +585,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
     const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch;
     if (!wavefront && p->max_bounces > 0) {
-        const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : kDefaultWavesPerEu;
+        const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : default_waves_per_eu(s->compact);
         const bool work = O.ray_work != nullptr;
         if (wpe != 2 && wpe != 3 && wpe != 4) throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
         if (s->compact && work) throw std::runtime_error("bbrt_trace: ray_work is not recorded on a compact scene");
@@ -1179,7 +1179,7 @@ int bbrt_autotune(bbrt_scene s, const double* origins, const double* dirs, const
         // coordinate descent from the library defaults over occupancy, Newton batch, block size and grid size
         bbrt_params cur = *base;
         cur.flags &= ~(uint32_t)BBRT_FLAG_GRAPH;
-        cur.block = 256; cur.waves_per_eu = kDefaultWavesPerEu; cur.newton_batch = kDefaultNewtonBatch;
+        cur.block = 256; cur.waves_per_eu = default_waves_per_eu(s->compact); cur.newton_batch = kDefaultNewtonBatch;
         cur.grid_fraction = 1.0f;
         double cur_ms = run(cur);
         auto sweep = [&](auto setter, std::initializer_list<double> values) {

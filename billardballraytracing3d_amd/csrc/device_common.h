@@ -15,6 +15,8 @@ namespace {
 
 constexpr int kStackCap = 48;
 constexpr int kDefaultWavesPerEu = 3;         // persistent-kernel occupancy target (best of 2/3/4 on MI355X)
+constexpr int kDefaultWavesPerEuCompact = 2;  // compact scenes: the on-the-fly cell evaluation wants the registers
+inline int default_waves_per_eu(bool compact) { return compact ? kDefaultWavesPerEuCompact : kDefaultWavesPerEu; }
 constexpr int kDefaultNewtonBatch = 64;       // parked lanes (stage-1 hits) that trigger a batched Newton (64: when no lane can go on)
 constexpr double kMtEps = 1e-9;               // MollerTrumbore.hpp:33
 constexpr double kOnNodeTol = 1.0e-12;        // PatchInterpolation.hpp:21
@@ -197,49 +199,70 @@ struct CellData {
 
 // Coarse vertex (i, j): (sum_m (sum_q ru[i][q] f[q][m]) rv[j][m]) * (su[i] sv[j]) per coordinate,
 // first-form interpolation (PatchInterpolation.hpp:54-159) with the inner index summed upwards.
-// N > 0: N x N nodes known at compile time.
+// The four corners of a cell are evaluated together, one coordinate at a
+// time: the inner sums over q for rows i and i+1 are shared by the corners of
+// equal i.  Each corner's operations and their order are those of the
+// single-vertex form, so the values are bit-identical.
+// N > 0: N x N nodes known at compile time; the loops unroll, so one
+// coordinate's nodal values are loaded together (one memory round trip).
+// N == 0: any node counts, plain loops.
 template <int N>
-__device__ __forceinline__ V3 coarse_point(const DevScene& S, const PatchMeta& M, int i, int j) {
-    const int im = N > 0 ? N : M.imax, jm = N > 0 ? N : M.jmax, nn = im * jm;
-    const double* ru = S.ff_rows + ((size_t)(M.nodeset >> 16) * bbrt::kCoarseN + i) * bbrt::kRowStride;
-    const double* rv = S.ff_rows + ((size_t)(M.nodeset & 0xFFFF) * bbrt::kCoarseN + j) * bbrt::kRowStride;
-    const double* F = S.fields + M.field_off;
-    const double sc = ru[bbrt::kMaxNodes1D] * rv[bbrt::kMaxNodes1D];
-    constexpr int L = N > 0 ? N : bbrt::kMaxNodes1D;
-    double out[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const double* f = F + c * nn;
-        double acc2 = 0.0;
+__device__ __forceinline__ void corner_coord(const double* __restrict__ f, const double* __restrict__ ru0,
+                                             const double* __restrict__ ru1, const double* __restrict__ rv0,
+                                             const double* __restrict__ rv1, int im_, int jm_, double a[4]) {
+    const int im = N > 0 ? N : im_, jm = N > 0 ? N : jm_;
+    constexpr int L = N > 0 ? N : 1;
+    double a00 = 0.0, a10 = 0.0, a01 = 0.0, a11 = 0.0;
+    if constexpr (N > 0) {
 #pragma unroll
         for (int m = 0; m < L; ++m) {
-            if (m < jm) {
-                double acc = 0.0;
+            double c0 = 0.0, c1 = 0.0;
 #pragma unroll
-                for (int q = 0; q < L; ++q)
-                    if (q < im) acc += ru[q] * f[q * jm + m];
-                acc2 += acc * rv[m];
+            for (int q = 0; q < L; ++q) {
+                const double fv = f[q * L + m];
+                c0 += ru0[q] * fv;
+                c1 += ru1[q] * fv;
             }
+            const double v0 = rv0[m], v1 = rv1[m];
+            a00 += c0 * v0; a10 += c1 * v0; a01 += c0 * v1; a11 += c1 * v1;
         }
-        out[c] = acc2 * sc;
+    } else {
+        for (int m = 0; m < jm; ++m) {
+            double c0 = 0.0, c1 = 0.0;
+            for (int q = 0; q < im; ++q) {
+                const double fv = f[q * jm + m];
+                c0 += ru0[q] * fv;
+                c1 += ru1[q] * fv;
+            }
+            const double v0 = rv0[m], v1 = rv1[m];
+            a00 += c0 * v0; a10 += c1 * v0; a01 += c0 * v1; a11 += c1 * v1;
+        }
     }
-    return { out[0], out[1], out[2] };
+    a[0] = a00; a[1] = a10; a[2] = a01; a[3] = a11;
 }
 
+// field value at the cell midpoint: sum_i lu[i] (sum_j lv[j] f[i][j]), rows with lu[i] == 0 skipped
+// (scene_build.cpp compute_cells does the same).
 template <int N>
 __device__ __forceinline__ double field_tab(const double* lu, const double* lv, const double* f, int im_, int jm_) {
     const int im = N > 0 ? N : im_, jm = N > 0 ? N : jm_;
-    constexpr int L = N > 0 ? N : bbrt::kMaxNodes1D;
+    constexpr int L = N > 0 ? N : 1;
     double sum = 0.0;
+    if constexpr (N > 0) {
 #pragma unroll
-    for (int i = 0; i < L; ++i) {
-        if (i < im) {
+        for (int i = 0; i < L; ++i) {
+            const double li = lu[i];
+            double row = 0.0;
+#pragma unroll
+            for (int j = 0; j < L; ++j) row += lv[j] * f[i * L + j];
+            if (li != 0.0) sum += li * row;
+        }
+    } else {
+        for (int i = 0; i < im; ++i) {
             const double li = lu[i];
             if (li != 0.0) {
                 double row = 0.0;
-#pragma unroll
-                for (int j = 0; j < L; ++j)
-                    if (j < jm) row += lv[j] * f[i * jm + j];
+                for (int j = 0; j < jm; ++j) row += lv[j] * f[i * jm + j];
                 sum += li * row;
             }
         }
@@ -248,22 +271,54 @@ __device__ __forceinline__ double field_tab(const double* lu, const double* lv, 
 }
 
 // Corners, sagitta and normal of cell (i, j) (SurfacePatch.hpp:416-492).
+// The loops over coordinates and fields stay rolled (values rotate through
+// registers instead of being indexed), so at most one field's nodal values are
+// live at a time; the fully unrolled form spills in the persistent kernel.
 template <int N>
 __device__ CellData compact_cell(const DevScene& S, int64_t k, int i, int j) {
-    const PatchMeta M = S.meta[k];
+    const int32_t ns = S.meta[k].nodeset;
+    const int64_t foff = S.meta[k].field_off;
+    const int im = N > 0 ? N : S.meta[k].imax, jm = N > 0 ? N : S.meta[k].jmax, nn = im * jm;
+    const double* ru0 = S.ff_rows + ((size_t)(ns >> 16) * bbrt::kCoarseN + i) * bbrt::kRowStride;
+    const double* ru1 = ru0 + bbrt::kRowStride;
+    const double* rv0 = S.ff_rows + ((size_t)(ns & 0xFFFF) * bbrt::kCoarseN + j) * bbrt::kRowStride;
+    const double* rv1 = rv0 + bbrt::kRowStride;
+    const double* F = S.fields + foff;
+    const double* lu = S.mid_basis + ((size_t)(ns >> 16) * bbrt::kCells + i) * bbrt::kMaxNodes1D;
+    const double* lv = S.mid_basis + ((size_t)(ns & 0xFFFF) * bbrt::kCells + j) * bbrt::kMaxNodes1D;
     CellData cd;
-    cd.p00 = coarse_point<N>(S, M, i, j);
-    cd.p10 = coarse_point<N>(S, M, i + 1, j);
-    cd.p01 = coarse_point<N>(S, M, i, j + 1);
-    cd.p11 = coarse_point<N>(S, M, i + 1, j + 1);
-    const int im = M.imax, jm = M.jmax, nn = im * jm;
-    const double* lu = S.mid_basis + ((size_t)(M.nodeset >> 16) * bbrt::kCells + i) * bbrt::kMaxNodes1D;
-    const double* lv = S.mid_basis + ((size_t)(M.nodeset & 0xFFFF) * bbrt::kCells + j) * bbrt::kMaxNodes1D;
-    const double* F = S.fields + M.field_off;
-    const V3 tm = { field_tab<N>(lu, lv, F, im, jm), field_tab<N>(lu, lv, F + nn, im, jm),
-                    field_tab<N>(lu, lv, F + 2 * nn, im, jm) };
-    cd.n = unit3({ field_tab<N>(lu, lv, F + 9 * nn, im, jm), field_tab<N>(lu, lv, F + 10 * nn, im, jm),
-                   field_tab<N>(lu, lv, F + 11 * nn, im, jm) });
+    // position fields: the corners and the midpoint value from the same nodal values
+    double x00 = 0, y00 = 0, z00 = 0, x10 = 0, y10 = 0, z10 = 0, x01 = 0, y01 = 0, z01 = 0, x11 = 0, y11 = 0, z11 = 0;
+    double t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        double a[4];
+        corner_coord<N>(F + c * nn, ru0, ru1, rv0, rv1, im, jm, a);
+        const double tv = field_tab<N>(lu, lv, F + c * nn, im, jm);
+        x00 = y00; y00 = z00; z00 = a[0];
+        x10 = y10; y10 = z10; z10 = a[1];
+        x01 = y01; y01 = z01; z01 = a[2];
+        x11 = y11; y11 = z11; z11 = a[3];
+        t0 = t1; t1 = t2; t2 = tv;
+    }
+    {
+        const double u0 = ru0[bbrt::kMaxNodes1D], u1 = ru1[bbrt::kMaxNodes1D];
+        const double v0 = rv0[bbrt::kMaxNodes1D], v1 = rv1[bbrt::kMaxNodes1D];
+        const double s00 = u0 * v0, s10 = u1 * v0, s01 = u0 * v1, s11 = u1 * v1;
+        cd.p00 = { x00 * s00, y00 * s00, z00 * s00 };
+        cd.p10 = { x10 * s10, y10 * s10, z10 * s10 };
+        cd.p01 = { x01 * s01, y01 * s01, z01 * s01 };
+        cd.p11 = { x11 * s11, y11 * s11, z11 * s11 };
+    }
+    // the normal fields (9-11) at the midpoint
+    double n0 = 0, n1 = 0, n2 = 0;
+#pragma unroll 1
+    for (int e = 9; e < 12; ++e) {
+        const double v = field_tab<N>(lu, lv, F + e * nn, im, jm);
+        n0 = n1; n1 = n2; n2 = v;
+    }
+    const V3 tm = { t0, t1, t2 };
+    cd.n = unit3({ n0, n1, n2 });
     const V3 bm = { 0.25 * (cd.p00.x + cd.p10.x + cd.p01.x + cd.p11.x), 0.25 * (cd.p00.y + cd.p10.y + cd.p01.y + cd.p11.y),
                     0.25 * (cd.p00.z + cd.p10.z + cd.p01.z + cd.p11.z) };
     const V3 dd = sub3(tm, bm);
@@ -351,14 +406,6 @@ __device__ __forceinline__ void cell_tests_data(const CellData& cd, int i, int j
     c.mt_tests += 6;
 }
 
-__device__ void stage1_brute_compact(const DevScene& S, int64_t k, const V3& o, const V3& d, Stage1& best, Counters& c) {
-    for (int i = 0; i < bbrt::kCells; ++i)
-        for (int j = 0; j < bbrt::kCells; ++j) {
-            const CellData cd = compact_cell_any(S, k, i, j);
-            cell_tests_data(cd, i, j, o, d, best, c);
-        }
-}
-
 // Stage 1 over the whole coarse grid in the reference's order (brute force).
 __device__ void stage1_brute(const double* G, const double* C, const V3& o, const V3& d, Stage1& best, Counters& c) {
     for (int i = 0; i < bbrt::kCells; ++i)
@@ -383,11 +430,9 @@ __device__ __forceinline__ uint32_t group_mask(const float* __restrict__ g, cons
     return m;
 }
 
-// Stage 1 through the culling pyramid (grouped layout: one load per level).
-// CM: 0 stored scenes only, 1 compact scenes only, 2 either (S.compact decides at run time).
-template <int CM>
+// Stage 1 through the culling pyramid (grouped layout: one load per level), stored scenes.
 __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, const double* C, const V3& o,
-                              const V3& d, const Ray32& r, double widen, Stage1& best, Counters& c) {
+                              const V3& d, const Ray32& r, Stage1& best, Counters& c) {
     const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
     const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
     const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
@@ -410,42 +455,6 @@ __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, con
                 const int tq = __ffs(mt) - 1;
                 mt &= mt - 1;
                 const int ti = 3 * bi + tq / 3, tj = 3 * bj + tq % 3;
-                if (CM == 1 || (CM == 2 && S.compact)) {
-                    // the quantized boxes (supersets) select cells; those are evaluated from the
-                    // fields, their exact box rebuilt and tested as stored, then Moller-Trumbore
-                    const float* tb = gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9 + 6 * tq;
-                    const uint2* q8 = reinterpret_cast<const uint2*>(
-                        S.qbox_cell + (size_t)k * bbrt::kQCellPatchBytes + (size_t)(ti * bbrt::kTileN + tj) * bbrt::kQCellBytes);
-                    uint8_t qb[56];
-#pragma unroll
-                    for (int w = 0; w < 7; ++w) {
-                        const uint2 v = q8[w];
-                        for (int b8 = 0; b8 < 4; ++b8) { qb[8 * w + b8] = (uint8_t)(v.x >> (8 * b8)); qb[8 * w + 4 + b8] = (uint8_t)(v.y >> (8 * b8)); }
-                    }
-                    const double tlo[3] = { tb[0], tb[1], tb[2] };
-                    const double step[3] = { ((double)tb[3] - tlo[0]) / 255.0, ((double)tb[4] - tlo[1]) / 255.0,
-                                             ((double)tb[5] - tlo[2]) / 255.0 };
-                    uint32_t mc = 0;
-                    for (int cq = 0; cq < 9; ++cq) {
-                        const uint8_t* qc = qb + 6 * cq;
-                        const float bx[6] = { round_down_f(tlo[0] + qc[0] * step[0]), round_down_f(tlo[1] + qc[1] * step[1]),
-                                              round_down_f(tlo[2] + qc[2] * step[2]), round_up_f(tlo[0] + qc[3] * step[0]),
-                                              round_up_f(tlo[1] + qc[4] * step[1]), round_up_f(tlo[2] + qc[5] * step[2]) };
-                        if (box32(bx, r)) mc |= 1u << cq;
-                    }
-                    c.cull_tests += 9;
-                    const PatchMeta M = S.meta[k];
-                    const int nn_kind = (M.imax == 3 && M.jmax == 3) ? 3 : ((M.imax == 5 && M.jmax == 5) ? 5 : 0);
-                    while (mc) {
-                        const int cq = __ffs(mc) - 1;
-                        mc &= mc - 1;
-                        const int i = 3 * ti + cq / 3, j = 3 * tj + cq % 3;
-                        const CellData cd = nn_kind == 5 ? compact_cell<5>(S, k, i, j)
-                                                         : (nn_kind == 3 ? compact_cell<3>(S, k, i, j) : compact_cell<0>(S, k, i, j));
-                        if (compact_cell_box(cd, widen, r)) cell_tests_data(cd, i, j, o, d, best, c);
-                    }
-                    continue;
-                }
                 uint32_t mc = group_mask<9, bbrt::kG9>(gc + (ti * bbrt::kTileN + tj) * bbrt::kG9, r);
                 c.cull_tests += 9;
                 while (mc) {
@@ -454,6 +463,96 @@ __device__ void stage1_culled(const DevScene& S, int64_t k, const double* G, con
                     cell_tests(G, C, 3 * ti + cq / 3, 3 * tj + cq % 3, o, d, best, c);
                 }
             }
+        }
+    }
+}
+
+// Cells of tile (ti, tj) whose quantized box (a superset of the exact FP32 box) the ray enters.
+__device__ __forceinline__ uint32_t qcell_mask(const DevScene& S, int64_t k, const float* tb, int ti, int tj,
+                                               const Ray32& r) {
+    const uint2* q8 = reinterpret_cast<const uint2*>(S.qbox_cell + (size_t)k * bbrt::kQCellPatchBytes +
+                                                     (size_t)(ti * bbrt::kTileN + tj) * bbrt::kQCellBytes);
+    uint32_t qw[14];   // the 56 bytes as little-endian words
+#pragma unroll
+    for (int w = 0; w < 7; ++w) {
+        const uint2 v = q8[w];
+        qw[2 * w] = v.x; qw[2 * w + 1] = v.y;
+    }
+    const double tlo[3] = { tb[0], tb[1], tb[2] };
+    const double step[3] = { ((double)tb[3] - tlo[0]) / 255.0, ((double)tb[4] - tlo[1]) / 255.0,
+                             ((double)tb[5] - tlo[2]) / 255.0 };
+    uint32_t mc = 0;
+#pragma unroll
+    for (int cq = 0; cq < 9; ++cq) {
+        double qv[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            const int byte = 6 * cq + e;
+            qv[e] = (double)((qw[byte >> 2] >> (8 * (byte & 3))) & 0xFFu);
+        }
+        const float bx[6] = { round_down_f(tlo[0] + qv[0] * step[0]), round_down_f(tlo[1] + qv[1] * step[1]),
+                              round_down_f(tlo[2] + qv[2] * step[2]), round_up_f(tlo[0] + qv[3] * step[0]),
+                              round_up_f(tlo[1] + qv[4] * step[1]), round_up_f(tlo[2] + qv[5] * step[2]) };
+        if (box32(bx, r)) mc |= 1u << cq;
+    }
+    return mc;
+}
+
+// Stage 1 of a compact scene.  Culled: the pyramid's super, block and tile
+// boxes select tiles, the quantized boxes select cells, each selected cell is
+// evaluated from the fields and its exact FP32 box rebuilt and tested as a
+// stored scene's, then Moller-Trumbore.  Not culled: every cell, tile by tile.
+// One loop yields the tiles in both cases, so the cell evaluation has a single
+// (inlined) site in the kernel.  Tiles are visited in the stored path's order;
+// ties between equal t are broken by the cell key, so the result does not
+// depend on the order anyway.
+__device__ void stage1_compact(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r, double widen,
+                               bool use_cull, Stage1& best, Counters& c) {
+    const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
+    const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
+    const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
+    uint32_t ms = 0, mb = 0, mt = 0;
+    int si = 0, sj = 0, bi = 0, bj = 0, t = 0;
+    if (use_cull) {
+        ms = group_mask<9, bbrt::kG9>(gs, r);
+        c.cull_tests += 9;
+    }
+    for (;;) {
+        int ti, tj;
+        uint32_t mc;
+        if (use_cull) {
+            while (mt == 0) {
+                while (mb == 0) {
+                    if (ms == 0) return;
+                    const int sq = __ffs(ms) - 1;
+                    ms &= ms - 1;
+                    si = sq / 3; sj = sq % 3;
+                    mb = group_mask<16, bbrt::kG16>(gb + sq * bbrt::kG16, r);
+                    c.cull_tests += 16;
+                }
+                const int bq = __ffs(mb) - 1;
+                mb &= mb - 1;
+                bi = bbrt::kBlocksPerSuper * si + (bq >> 2); bj = bbrt::kBlocksPerSuper * sj + (bq & 3);
+                mt = group_mask<9, bbrt::kG9>(gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9, r);
+                c.cull_tests += 9;
+            }
+            const int tq = __ffs(mt) - 1;
+            mt &= mt - 1;
+            ti = 3 * bi + tq / 3; tj = 3 * bj + tq % 3;
+            mc = qcell_mask(S, k, gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9 + 6 * tq, ti, tj, r);
+            c.cull_tests += 9;
+        } else {
+            if (t == bbrt::kTileN * bbrt::kTileN) return;
+            ti = t / bbrt::kTileN; tj = t % bbrt::kTileN;
+            ++t;
+            mc = 0x1FFu;
+        }
+        while (mc) {
+            const int cq = __ffs(mc) - 1;
+            mc &= mc - 1;
+            const int i = 3 * ti + cq / 3, j = 3 * tj + cq % 3;
+            const CellData cd = compact_cell_any(S, k, i, j);
+            if (!use_cull || compact_cell_box(cd, widen, r)) cell_tests_data(cd, i, j, o, d, best, c);
         }
     }
 }
@@ -649,8 +748,9 @@ __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const 
     const double* C = S.cell + (size_t)k * bbrt::kCells * bbrt::kCells * 4;
     Stage1 best{ 0.0, 0, false };
     c.candidates++;
-    if (cull && S.meta[k].cull) stage1_culled<CM>(S, k, G, C, o, d, r32, widen, best, c);
-    else if (CM == 1 || (CM == 2 && S.compact)) stage1_brute_compact(S, k, o, d, best, c);
+    const bool use_cull = cull && S.meta[k].cull;
+    if (CM == 1 || (CM == 2 && S.compact)) stage1_compact(S, k, o, d, r32, widen, use_cull, best, c);
+    else if (use_cull) stage1_culled(S, k, G, C, o, d, r32, best, c);
     else stage1_brute(G, C, o, d, best, c);
     Seed sd{ 0.0, 0.0, best.t, best.have };
     if (!best.have) return sd;

@@ -131,17 +131,22 @@ def sphere_triangles(kind: str, freq: int, rot: Optional[np.ndarray] = None) -> 
     f = _orient_outward(v, f)
     if rot is not None:
         v = v @ rot.T
-    tris = []
-    for A, B, C in v[f]:
-        def P(i, j):
-            p = A * (freq - i - j) / freq + B * i / freq + C * j / freq
-            return p / np.linalg.norm(p)
-        for i in range(freq):
-            for j in range(freq - i):
-                tris.append((P(i, j), P(i + 1, j), P(i, j + 1)))
-                if i + j < freq - 1:
-                    tris.append((P(i + 1, j), P(i + 1, j + 1), P(i, j + 1)))
-    return np.asarray(tris, dtype=np.float64)
+    # corner (i, j) of the subdivision, per face; triangles in the order "up (i,j), then down (i,j)"
+    idx = []
+    for i in range(freq):
+        for j in range(freq - i):
+            idx.append(((i, j), (i + 1, j), (i, j + 1)))
+            if i + j < freq - 1:
+                idx.append(((i + 1, j), (i + 1, j + 1), (i, j + 1)))
+    ij = np.asarray(idx, dtype=np.int64)                             # (t, 3, 2)
+    wi = ij[..., 0].astype(np.float64)[None, :, :, None]
+    wj = ij[..., 1].astype(np.float64)[None, :, :, None]
+    w0 = (freq - ij[..., 0] - ij[..., 1]).astype(np.float64)[None, :, :, None]
+    F = v[f]                                                         # (faces, 3, 3)
+    A, B, C = (F[:, k][:, None, None, :] for k in range(3))
+    p = A * w0 / freq + B * wi / freq + C * wj / freq                # (faces, t, 3, 3)
+    p = p / np.sqrt((p * p).sum(axis=-1, keepdims=True))
+    return np.ascontiguousarray(p.reshape(-1, 3, 3), dtype=np.float64)
 
 
 # --------------------------------------------------------------------------

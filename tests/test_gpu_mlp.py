@@ -6,6 +6,8 @@
 * the bounce loop with the learned seed gives the triangle-seed results (patch sequences exact, points within
   1e-10) -- with a trained network and with an untrained one, whose seeds all fall back.
 """
+from types import SimpleNamespace
+
 import numpy as np
 import pytest
 
@@ -176,3 +178,25 @@ def test_learned_seed_argument_checks(c1):
     bad[M.OFF_W1 + 60] = 1.0   # a pad of W1
     with pytest.raises(L.BBRTError, match="padding"):
         net.params = bad
+
+
+def test_batched_compact_learned_matches_triangle(trained):
+    """The batched form at small scale: compact, device-built sub-scenes, learned seeds with fallback; the
+    records equal the triangle-seed trace of the same handle, sub-scene 0 equals the oracle, and the per-ray
+    learned/fallback counts add up to the counters."""
+    net, _, _ = trained
+    scenes, o, d, rb = S.batch_config(seed=8, n_scenes=5, patch_lo=80, patch_hi=600, rays_lo=300, rays_hi=1500)
+    gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
+    gb.set_mlp(net)
+    mb = 25
+    gl = gb.trace(o, d, mb, mode=L.NEWTON_CONVERGED, seed=L.SEED_LEARNED, ray_begin=rb, seed_stats=True)
+    tri = gb.trace(o, d, mb, mode=L.NEWTON_CONVERGED, ray_begin=rb)
+    _compare(gl, tri)
+    n_hits = gl.counter("seed_learned") + gl.counter("seed_fallback")
+    assert n_hits == tri.counter("newton_calls") > 0
+    assert gl.seed_stats[:, 0].sum() == gl.counter("seed_learned")
+    assert gl.seed_stats[:, 1].sum() == gl.counter("seed_fallback")
+    assert gl.counter("seed_learned") > 0
+    sl = slice(rb[0], rb[1])
+    part = SimpleNamespace(**{f: getattr(gl, f)[sl] for f in ("status", "err_kind", "bounces", "hit_patch", "hit_xyz")})
+    _compare(part, O.OracleScene(scenes[0]).trace(o[sl], d[sl], mb, mode=L.NEWTON_CONVERGED))
