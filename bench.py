@@ -55,6 +55,7 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="run the batched-scenes leg alone")
     ap.add_argument("--workload", choices=["sinai", "c5"], default="sinai",
                     help="c5: the full batched form (512 scenes, 16M rays, 200 bounces, learned seeds) alone")
+    ap.add_argument("--no-full-form", action="store_true", help="skip the full batched form leg (512 scenes, 16M rays)")
     ap.add_argument("--c5-scenes", type=int, default=512)
     ap.add_argument("--c5-rays", type=int, default=16_000_000)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -176,17 +177,23 @@ def main():
             res["learned"] = learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value, res)
             res["mean_newton_iters"]["learned"] = res["learned"]["mean_newton_iters"]
 
-    if not args.no_batch:
-        del to, td, outs
-        torch.cuda.empty_cache()
-        res["batch"] = batch_leg(args)
-
     traffic = committed_traffic(args, n, mb, K)
     if traffic is not None:
         res["roofline"]["traffic"] = traffic
         res["roofline"]["traffic_source"] = "profiles/k_persist_sinai_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)"
     if args.cpu_seconds > 0:
         res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, mode, args.cpu_seconds, gs, K)
+
+    # the batched legs need the HBM the Sinai scene and its buffers hold
+    del to, td, outs, ptrs, gs
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        _progress("batched bounce map leg")
+        res["batch"] = batch_leg(args)
+        torch.cuda.empty_cache()
+    if not args.no_full_form:
+        _progress("full batched form leg")
+        res["full_form"] = c5_leg(args)
     print(json.dumps(res), flush=True)
 
 
