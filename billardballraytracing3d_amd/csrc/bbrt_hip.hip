@@ -158,9 +158,11 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
     double inv[3] = { 0, 0, 0 };
     bool cull = false;
     Ray32 r32 = { 0, 0, 0, 0, 0, 0 };
-    Ent stack[kStackCap];
+    Ent4 stack[kStack4Cap];
     int sp = 0;
-    Ent cur = { -1, 0, 0.0 };
+    int32_t cur = -1;                         // 4-wide node (>= 0), leaf (-2 - binary node) or nothing (-1)
+    float cur_t = 0.0f;                       // its entry t (FP32, prune only)
+    uint32_t negm = 0;                        // direction sign bits: the depth-first order
     double best_t = INFINITY;
     PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     int32_t best_obj = -1, err_obj = -1;
@@ -178,11 +180,12 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         inv[0] = 1.0 / d.x; inv[1] = 1.0 / d.y; inv[2] = 1.0 / d.z;
         best_t = INFINITY;
         best_obj = -1; err_obj = -1; err = 0; sp = 0;
+        negm = (inv[0] < 0.0 ? 1u : 0u) | (inv[1] < 0.0 ? 2u : 0u) | (inv[2] < 0.0 ? 4u : 0u);
         const int32_t ri = S.scenes[scn].root;
-        const BvhNode root = S.bvh[ri];
         double tm = -INFINITY;
-        const bool ok = root_live ? true : box_static(o, inv, root, tm);
-        cur = make_ent(ri, root, ok, tm);
+        const bool ok = root_live ? true : box_static(o, inv, S.bvh[ri], tm);
+        cur = ok ? ri : -1;                   // the root's 4-wide node (built for a leaf root too)
+        cur_t = -INFINITY;
         phase = kPhTrav;
         if constexpr (WORK) c0 = c;
     };
@@ -211,47 +214,97 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         }
         if (__ballot(r >= 0) == 0) break;
 
-        // traversal: internal nodes until a leaf to intersect, or the end (BVH.hpp:91-126)
+        // traversal: 4-wide nodes until a leaf to intersect, or the end (BVH.hpp:91-126).
+        // Slots are tested in FP32 (a superset of the binary nodes the FP64 walk
+        // enters) and visited in the binary walk's depth-first order; a leaf is
+        // intersected only if its own FP64 test with the current best_t passes,
+        // which is exactly the reference's condition for popping it live (the
+        // ancestors' tests pass whenever the leaf's does: their boxes contain it and
+        // their entry t is no larger).  So the leaves intersected, their order and
+        // every record are the binary walk's.  Rays whose origin is beyond the
+        // culling limit test the slots' binary nodes in FP64 instead.
         if (r >= 0 && phase == kPhTrav) {
             for (;;) {
-                c.bvh_tests++;
-                const bool live_node = cur.idx >= 0 && !(cur.tmin > best_t);
-                if (live_node && cur.meta < 0) {
-                    const int v = -1 - cur.meta;
-                    leaf_start = v >> 3; leaf_count = v & 7; q = 0;
-                    // the conservative box of each object's culling pyramid, for the whole leaf at once
-                    enter = 0;
-                    if (cull) {
-                        for (int k = 0; k < leaf_count; ++k) {
-                            const int32_t obj = S.leaf[leaf_start + k];
-                            const float4* b4 = reinterpret_cast<const float4*>(S.box_patch + (size_t)obj * 8);
-                            const float4 a = b4[0], bb = b4[1];
-                            const float bx[6] = { a.x, a.y, a.z, a.w, bb.x, bb.y };
-                            if (box32(bx, r32)) enter |= 1u << k;
-                        }
-                        c.cull_tests += leaf_count;
-                    } else {
-                        enter = 0x7Fu;
-                    }
-                    phase = kPhLeaf;
-                    break;
-                }
-                if (!live_node) {
+                if (cur == -1 || (double)cur_t > best_t * kPruneSlack) {
                     if (sp == 0) { phase = kPhDone; break; }
-                    cur = stack[--sp];
+                    --sp;
+                    cur = stack[sp].ref; cur_t = stack[sp].tmin;
                     continue;
                 }
-                const int32_t left = cur.idx + 1, right = cur.meta >> 2;
-                const BvhNode nl = S.bvh[left];
-                const BvhNode nr = S.bvh[right];
-                double tl = 0.0, tr = 0.0;
-                const bool okl = box_static(o, inv, nl, tl);
-                const bool okr = box_static(o, inv, nr, tr);
-                if (sp + 1 > kStackCap) { err = bbrt::kErrStack; phase = kPhDone; break; }
-                const Ent el = make_ent(left, nl, okl, tl), er = make_ent(right, nr, okr, tr);
-                const bool negax = (cur.meta & 3) == 0 ? inv[0] < 0.0 : ((cur.meta & 3) == 1 ? inv[1] < 0.0 : inv[2] < 0.0);
-                if (negax) { stack[sp++] = el; cur = er; }
-                else { stack[sp++] = er; cur = el; }
+                c.bvh_tests++;
+                if (cur <= -2) {
+                    const BvhNode nd = S.bvh[-2 - cur];
+                    double tm = 0.0;
+                    if (box_static(o, inv, nd, tm) && !(tm > best_t)) {
+                        leaf_start = nd.start; leaf_count = nd.count; q = 0;
+                        // the conservative box of each object's culling pyramid, for the whole leaf at once
+                        enter = 0;
+                        if (cull) {
+                            for (int k = 0; k < leaf_count; ++k) {
+                                const int32_t obj = S.leaf[leaf_start + k];
+                                const float4* b4 = reinterpret_cast<const float4*>(S.box_patch + (size_t)obj * 8);
+                                const float4 a = b4[0], bb = b4[1];
+                                const float bx[6] = { a.x, a.y, a.z, a.w, bb.x, bb.y };
+                                if (box32(bx, r32)) enter |= 1u << k;
+                            }
+                            c.cull_tests += leaf_count;
+                        } else {
+                            enter = 0x7Fu;
+                        }
+                        phase = kPhLeaf;
+                        break;
+                    }
+                    cur = -1;
+                    continue;
+                }
+                const bbrt::Bvh4Node* N = S.bvh4 + cur;
+                const int4 rf = *reinterpret_cast<const int4*>(N->ref);
+                const int axes = N->axes;
+                const int32_t ref[4] = { rf.x, rf.y, rf.z, rf.w };
+                float tmn[4];
+                uint32_t hm;
+                if (cull) {
+                    hm = slots32(N, r32, tmn);
+                } else {
+                    hm = 0;
+#pragma unroll
+                    for (int sl = 0; sl < 4; ++sl) {
+                        tmn[sl] = 0.0f;
+                        if (ref[sl] != -1) {
+                            double tm = 0.0;
+                            if (box_static(o, inv, S.bvh[ref[sl] >= 0 ? ref[sl] : -2 - ref[sl]], tm)) {
+                                hm |= 1u << sl;
+                                tmn[sl] = (float)tm;
+                            }
+                        }
+                    }
+                }
+                hm &= (ref[0] != -1 ? 1u : 0u) | (ref[1] != -1 ? 2u : 0u) | (ref[2] != -1 ? 4u : 0u) |
+                      (ref[3] != -1 ? 8u : 0u);
+                // depth-first order of the slots: the node's split axis picks the pair, each
+                // child's axis the slot within it (BVH.hpp:118-125)
+                const int f0 = (negm >> (axes & 3)) & 1, fl = (negm >> ((axes >> 2) & 3)) & 1,
+                          fr = (negm >> ((axes >> 4) & 3)) & 1;
+                const int l0 = fl, l1 = 1 - fl, r0 = 2 + fr, r1 = 3 - fr;
+                const int sq[4] = { f0 ? r0 : l0, f0 ? r1 : l1, f0 ? l0 : r0, f0 ? l1 : r1 };
+                int32_t R[4];
+                float T[4];
+                uint32_t pm = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int sl = sq[k];
+                    R[k] = sl == 0 ? ref[0] : (sl == 1 ? ref[1] : (sl == 2 ? ref[2] : ref[3]));
+                    T[k] = sl == 0 ? tmn[0] : (sl == 1 ? tmn[1] : (sl == 2 ? tmn[2] : tmn[3]));
+                    pm |= ((hm >> sl) & 1u) << k;
+                }
+                if (pm == 0) { cur = -1; continue; }
+                if (sp + 3 > kStack4Cap) { err = bbrt::kErrStack; phase = kPhDone; break; }
+                const int fq = __ffs(pm) - 1;
+#pragma unroll
+                for (int k = 3; k >= 1; --k)
+                    if (k > fq && ((pm >> k) & 1u)) { stack[sp].ref = R[k]; stack[sp].tmin = T[k]; ++sp; }
+                cur = fq == 0 ? R[0] : (fq == 1 ? R[1] : (fq == 2 ? R[2] : R[3]));
+                cur_t = fq == 0 ? T[0] : (fq == 1 ? T[1] : (fq == 2 ? T[2] : T[3]));
             }
         }
 
@@ -298,10 +351,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         if (have_h) {
             if (h.status < 0) { err = -h.status; err_obj = obj; phase = kPhDone; }
             else if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
-            if (phase == kPhLeaf && ++q == leaf_count) {
-                if (sp == 0) phase = kPhDone;
-                else { cur = stack[--sp]; phase = kPhTrav; }
-            }
+            if (phase == kPhLeaf && ++q == leaf_count) { cur = -1; phase = kPhTrav; }   // next entry
         }
 
         if (r >= 0 && phase == kPhDone) {
@@ -873,6 +923,14 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
             if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
                 throw std::runtime_error(std::string(who) + ": BVH exceeds the packed node limits");
         D.bvh = dev_upload(bvh, s->allocs, s->bytes);
+        {
+            std::vector<int32_t> roots;
+            std::vector<double> widen;
+            for (const auto& si : s->scenes) { roots.push_back(si.root); widen.push_back(si.cull_widen); }
+            std::vector<bbrt::Bvh4Node> bvh4;
+            bbrt::build_bvh4(bvh, roots, widen, bvh4);
+            D.bvh4 = dev_upload(bvh4, s->allocs, s->bytes);
+        }
         D.leaf = dev_upload(leaf, s->allocs, s->bytes);
         s->d_scenes = const_cast<bbrt::SceneInfo*>(dev_upload(s->scenes, s->allocs, s->bytes));
         D.scenes = s->d_scenes;

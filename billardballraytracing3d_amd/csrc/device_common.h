@@ -14,6 +14,8 @@ using bbrt::PatchMeta;
 namespace {
 
 constexpr int kStackCap = 48;
+constexpr int kStack4Cap = 64;                // 4-wide traversal stack (3 entries per level; depth <= 21 levels)
+constexpr double kPruneSlack = 1.000001;      // FP32 tmin prune: relative slack over the FP64 best_t
 constexpr int kDefaultWavesPerEu = 3;         // persistent-kernel occupancy target (best of 2/3/4 on MI355X)
 constexpr int kDefaultWavesPerEuCompact = 2;  // compact scenes: the on-the-fly cell evaluation wants the registers
 inline int default_waves_per_eu(bool compact) { return compact ? kDefaultWavesPerEuCompact : kDefaultWavesPerEu; }
@@ -42,6 +44,7 @@ struct DevScene {
     const float* box_super;
     const float* box_patch;
     const BvhNode* bvh;
+    const bbrt::Bvh4Node* bvh4;      // per binary node: its 4-wide node (filled at even depth)
     const int32_t* leaf;
     const bbrt::SceneInfo* scenes;   // per sub-scene: BVH root, first patch, diagonal, culling limit
     const int32_t* ray_scene;        // per ray: its sub-scene (NULL: every ray in sub-scene 0)
@@ -147,6 +150,33 @@ __device__ __forceinline__ bool box32(const float* __restrict__ b, const Ray32& 
     if (r.iz < 0.0f) { const float s = t1; t1 = t2; t2 = s; }
     tmin = (tmin < t1) ? t1 : tmin; tmax = (t2 < tmax) ? t2 : tmax;
     return tmax >= tmin && tmax >= 0.0f;
+}
+
+// FP32 ray test of the four slots of a 4-wide node: the mask of slots whose
+// (widened) box the ray enters at t >= 0, and their entry t.  Widening and
+// outward rounding make it a superset of the FP64 test of the slots' binary
+// nodes for rays with |origin| within the culling limit.
+__device__ __forceinline__ uint32_t slots32(const bbrt::Bvh4Node* __restrict__ N, const Ray32& r, float tmin_out[4]) {
+    const float4* p = reinterpret_cast<const float4*>(N);
+    const float4 lx = p[0], ly = p[1], lz = p[2], hx = p[3], hy = p[4], hz = p[5];
+    const float lo[3][4] = { { lx.x, lx.y, lx.z, lx.w }, { ly.x, ly.y, ly.z, ly.w }, { lz.x, lz.y, lz.z, lz.w } };
+    const float hi[3][4] = { { hx.x, hx.y, hx.z, hx.w }, { hy.x, hy.y, hy.z, hy.w }, { hz.x, hz.y, hz.z, hz.w } };
+    const float oo[3] = { r.ox, r.oy, r.oz }, ii[3] = { r.ix, r.iy, r.iz };
+    uint32_t m = 0;
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl) {
+        float tmin = -INFINITY, tmax = INFINITY;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float t1 = (lo[a][sl] - oo[a]) * ii[a], t2 = (hi[a][sl] - oo[a]) * ii[a];
+            if (ii[a] < 0.0f) { const float t = t1; t1 = t2; t2 = t; }
+            tmin = (tmin < t1) ? t1 : tmin;
+            tmax = (t2 < tmax) ? t2 : tmax;
+        }
+        tmin_out[sl] = tmin;
+        if (tmax >= tmin && tmax >= 0.0f) m |= 1u << sl;
+    }
+    return m;
 }
 
 struct Stage1 {
@@ -843,6 +873,12 @@ __device__ __forceinline__ bool box_static(const V3& o, const double inv[3], con
 // Traversal stack entry: idx < 0 marks a node whose box test already failed;
 // meta >= 0 is an internal node (right child << 2 | split axis), meta < 0 a
 // leaf (-1 - (start << 3 | count)); tmin is the node's entry distance.
+// 4-wide traversal stack entry: node or leaf reference and its FP32 entry t.
+struct Ent4 {
+    int32_t ref;
+    float tmin;
+};
+
 struct Ent {
     int32_t idx;
     int32_t meta;

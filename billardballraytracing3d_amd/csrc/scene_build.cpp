@@ -611,4 +611,48 @@ void build_bvh(const std::vector<double>& box, const std::vector<double>& cent, 
     tb.build(std::move(objs));
 }
 
+void build_bvh4(const std::vector<BvhNode>& bvh, const std::vector<int32_t>& roots, const std::vector<double>& widen,
+                std::vector<Bvh4Node>& out) {
+    out.assign(bvh.size(), Bvh4Node{});
+    const float inf = std::numeric_limits<float>::infinity();
+    for (size_t i = 0; i < roots.size(); ++i) {
+        const double w = widen[i];
+        std::vector<int32_t> todo{ roots[i] };
+        while (!todo.empty()) {
+            const int32_t b = todo.back();
+            todo.pop_back();
+            Bvh4Node& q = out[b];
+            for (int s = 0; s < 4; ++s) {
+                q.ref[s] = -1;
+                for (int a = 0; a < 3; ++a) { q.lo[a][s] = inf; q.hi[a][s] = -inf; }
+            }
+            q.axes = 0;
+            auto put = [&](int s, int32_t k) {
+                const BvhNode& nd = bvh[k];
+                for (int a = 0; a < 3; ++a) { q.lo[a][s] = round_down(nd.lo[a] - w); q.hi[a][s] = round_up(nd.hi[a] + w); }
+                if (nd.right >= 0) { q.ref[s] = k; todo.push_back(k); }
+                else q.ref[s] = -2 - k;
+            };
+            const BvhNode& B = bvh[b];
+            if (B.right < 0) {          // a leaf root: one slot
+                put(0, b);
+                q.ref[0] = -2 - b;
+                continue;
+            }
+            const int32_t kids[2] = { b + 1, B.right };
+            q.axes = B.axis;
+            for (int h = 0; h < 2; ++h) {
+                const BvhNode& X = bvh[kids[h]];
+                if (X.right < 0) {
+                    put(2 * h, kids[h]);
+                } else {
+                    q.axes |= X.axis << (2 + 2 * h);
+                    put(2 * h, kids[h] + 1);
+                    put(2 * h + 1, X.right);
+                }
+            }
+        }
+    }
+}
+
 }  // namespace bbrt

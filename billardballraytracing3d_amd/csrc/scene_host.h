@@ -75,6 +75,21 @@ struct BvhNode {
     int32_t right, start, count, axis;
 };
 
+// 4-wide traversal node over the binary BVH, stored at the index of binary node
+// B: its slots are B's grandchildren (slots 0-1 under B's left child, 2-3 under
+// its right child; a leaf child takes one slot of its pair), so one load and
+// four box tests replace two levels of the binary walk.  The boxes are FP32,
+// rounded outward and widened, so a slot whose binary node passes the FP64 ray
+// test also passes the FP32 one; leaves are tested again in FP64 before use.
+// 128 bytes: [axis][slot] lo and hi, then the slots' binary node indices.
+struct Bvh4Node {
+    float lo[3][4], hi[3][4];
+    int32_t ref[4];   // >= 0: internal binary node (a 4-wide node), <= -2: leaf binary node -2 - ref, -1: empty
+    int32_t axes;     // split axes that fix the depth-first order: B | left child << 2 | right child << 4
+    int32_t pad[3];
+};
+static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node is two cache lines");
+
 struct HostScene {
     int64_t n = 0;
     std::vector<PatchMeta> meta;
@@ -136,6 +151,11 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& out, int threads, boo
                       bool deferred = false);
 void finish_host_scene(HostScene& S, const std::vector<double>& max_sag, const std::vector<int32_t>& finite,
                        const std::vector<double>& hmax);
+
+// The 4-wide nodes of every binary node at even depth below each root (roots[i],
+// its FP32 widening widen[i]); out has one entry per binary node.
+void build_bvh4(const std::vector<BvhNode>& bvh, const std::vector<int32_t>& roots, const std::vector<double>& widen,
+                std::vector<Bvh4Node>& out);
 
 // BVH::build (BVH.hpp:136-189) over objects with boxes [n][6] and centroids [n][3].
 void build_bvh(const std::vector<double>& box, const std::vector<double>& cent, int leaf_threshold,
