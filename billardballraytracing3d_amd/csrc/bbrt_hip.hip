@@ -233,24 +233,35 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 }
                 c.bvh_tests++;
                 if (cur <= -2) {
-                    const BvhNode nd = S.bvh[-2 - cur];
-                    double tm = 0.0;
-                    if (box_static(o, inv, nd, tm) && !(tm > best_t)) {
-                        leaf_start = nd.start; leaf_count = nd.count; q = 0;
-                        // the conservative box of each object's culling pyramid, for the whole leaf at once
-                        enter = 0;
-                        if (cull) {
-                            for (int k = 0; k < leaf_count; ++k) {
-                                const int32_t obj = S.leaf[leaf_start + k];
-                                const float4* b4 = reinterpret_cast<const float4*>(S.box_patch + (size_t)obj * 8);
-                                const float4 a = b4[0], bb = b4[1];
-                                const float bx[6] = { a.x, a.y, a.z, a.w, bb.x, bb.y };
-                                if (box32(bx, r32)) enter |= 1u << k;
-                            }
-                            c.cull_tests += leaf_count;
-                        } else {
-                            enter = 0x7Fu;
+                    // a leaf: its FP64 box, object ids and culling boxes are addressed by its start,
+                    // so the loads go out together
+                    const int v = -2 - cur;
+                    const int ls = v >> 3, lc = v & 7;
+                    const double2* lb2 = reinterpret_cast<const double2*>(S.leaf_box + 6 * (size_t)ls);
+                    const double2 b01 = lb2[0], b23 = lb2[1], b45 = lb2[2];
+                    uint32_t en = 0;
+                    if (cull) {
+                        for (int k = 0; k < lc; ++k) {
+                            const float4* b4 = reinterpret_cast<const float4*>(S.leaf_pbox + (size_t)(ls + k) * 8);
+                            const float4 a = b4[0], bb = b4[1];
+                            const float bx[6] = { a.x, a.y, a.z, a.w, bb.x, bb.y };
+                            if (box32(bx, r32)) en |= 1u << k;
                         }
+                    } else {
+                        en = (1u << lc) - 1u;
+                    }
+                    const double blo[3] = { b01.x, b01.y, b23.x }, bhi[3] = { b23.y, b45.x, b45.y };
+                    double tm = 0.0;
+                    if (box_static6(o, inv, blo, bhi, tm) && !(tm > best_t)) {
+                        if (cull) c.cull_tests += lc;
+                        if (en == 0) {            // no object's culling box is entered: no stage-1 hit here
+                            c.candidates += lc;
+                            cur = -1;
+                            continue;
+                        }
+                        leaf_start = ls; leaf_count = lc; enter = en;
+                        q = __ffs(en) - 1;        // objects before the first entered one: counted, skipped
+                        c.candidates += q;
                         phase = kPhLeaf;
                         break;
                     }
@@ -272,7 +283,11 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                         tmn[sl] = 0.0f;
                         if (ref[sl] != -1) {
                             double tm = 0.0;
-                            if (box_static(o, inv, S.bvh[ref[sl] >= 0 ? ref[sl] : -2 - ref[sl]], tm)) {
+                            const bool hit = ref[sl] >= 0
+                                ? box_static(o, inv, S.bvh[ref[sl]], tm)
+                                : box_static6(o, inv, S.leaf_box + 6 * (size_t)((-2 - ref[sl]) >> 3),
+                                              S.leaf_box + 6 * (size_t)((-2 - ref[sl]) >> 3) + 3, tm);
+                            if (hit) {
                                 hm |= 1u << sl;
                                 tmn[sl] = (float)tm;
                             }
@@ -351,7 +366,12 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         if (have_h) {
             if (h.status < 0) { err = -h.status; err_obj = obj; phase = kPhDone; }
             else if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
-            if (phase == kPhLeaf && ++q == leaf_count) { cur = -1; phase = kPhTrav; }   // next entry
+            if (phase == kPhLeaf) {            // the next entered object, or back to the walk
+                ++q;
+                const uint32_t rest = q < leaf_count ? (enter >> q) : 0u;
+                if (rest == 0) { c.candidates += leaf_count - q; cur = -1; phase = kPhTrav; }
+                else { const int nq = q + __ffs(rest) - 1; c.candidates += nq - q; q = nq; }
+            }
         }
 
         if (r >= 0 && phase == kPhDone) {
@@ -402,6 +422,15 @@ __global__ void k_pad_records(Outs O, int64_t total) {
         if (O.newton_iters) O.newton_iters[q] = 0;
         if (O.hit_xyz) { O.hit_xyz[3 * q] = 0.0; O.hit_xyz[3 * q + 1] = 0.0; O.hit_xyz[3 * q + 2] = 0.0; }
     }
+}
+
+// Culling boxes in leaf order: leaf_pbox[p] = box_patch[leaf[p]] (two float4 per box).
+__global__ void k_gather_leaf_pbox(const int32_t* __restrict__ leaf, const float* __restrict__ box_patch, int64_t n,
+                                   float* __restrict__ leaf_pbox) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 2 * n) return;
+    const int64_t p = g >> 1, h = g & 1;
+    reinterpret_cast<float4*>(leaf_pbox)[2 * p + h] = reinterpret_cast<const float4*>(box_patch)[2 * (int64_t)leaf[p] + h];
 }
 
 __global__ void k_zero_i32(int32_t* p, int n) {
@@ -932,6 +961,13 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
             D.bvh4 = dev_upload(bvh4, s->allocs, s->bytes);
         }
         D.leaf = dev_upload(leaf, s->allocs, s->bytes);
+        {
+            std::vector<double> lb(6 * std::max<size_t>(1, leaf.size()), 0.0);
+            for (const BvhNode& nd : bvh)
+                if (nd.right < 0)
+                    for (int a = 0; a < 3; ++a) { lb[6 * (size_t)nd.start + a] = nd.lo[a]; lb[6 * (size_t)nd.start + 3 + a] = nd.hi[a]; }
+            D.leaf_box = dev_upload(lb, s->allocs, s->bytes);
+        }
         s->d_scenes = const_cast<bbrt::SceneInfo*>(dev_upload(s->scenes, s->allocs, s->bytes));
         D.scenes = s->d_scenes;
         if (gpu_build) {
@@ -965,6 +1001,21 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
             D.box_block = upload_parts(&bbrt::HostScene::box_block);
             D.box_super = upload_parts(&bbrt::HostScene::box_super);
             D.box_patch = upload_parts(&bbrt::HostScene::box_patch);
+        }
+        {
+            float* lp = nullptr;
+            const size_t nl = std::max<size_t>(1, leaf.size());
+            check(hipMalloc(&lp, nl * 8 * sizeof(float)), "hipMalloc(scene)");
+            s->allocs.push_back(lp);
+            s->bytes += (int64_t)(nl * 8 * sizeof(float));
+            if (!leaf.empty()) {
+                const int64_t total = (int64_t)leaf.size() * 2;
+                hipLaunchKernelGGL(k_gather_leaf_pbox, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s->stream,
+                                   D.leaf, D.box_patch, (int64_t)leaf.size(), lp);
+                check(hipGetLastError(), "k_gather_leaf_pbox");
+                check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+            }
+            D.leaf_pbox = lp;
         }
         s->n_patches = n_patch;
         for (auto& h : H) {

@@ -46,6 +46,8 @@ struct DevScene {
     const BvhNode* bvh;
     const bbrt::Bvh4Node* bvh4;      // per binary node: its 4-wide node (filled at even depth)
     const int32_t* leaf;
+    const double* leaf_box;          // [leaf position][6]: FP64 box of the leaf whose objects start there
+    const float* leaf_pbox;          // [leaf position][8]: culling box of the patch at that leaf position
     const bbrt::SceneInfo* scenes;   // per sub-scene: BVH root, first patch, diagonal, culling limit
     const int32_t* ray_scene;        // per ray: its sub-scene (NULL: every ray in sub-scene 0)
     // compact scenes: no cxyz / cell / box_cell; cells are evaluated from the node-set tables
@@ -854,13 +856,14 @@ __device__ __forceinline__ void write_final(const Outs& O, int32_t r, const V3& 
 
 // The part of ray_box (BVH.hpp:14-37) that does not depend on best_t: false
 // when the box is rejected whatever best_t is, else the entry distance.
-__device__ __forceinline__ bool box_static(const V3& o, const double inv[3], const BvhNode& nd, double& tmin_out) {
+__device__ __forceinline__ bool box_static6(const V3& o, const double inv[3], const double lo[3], const double hi[3],
+                                            double& tmin_out) {
     double tmin = -INFINITY, tmax = INFINITY;
     const double oo[3] = { o.x, o.y, o.z };
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        double t1 = (nd.lo[a] - oo[a]) * inv[a];
-        double t2 = (nd.hi[a] - oo[a]) * inv[a];
+        double t1 = (lo[a] - oo[a]) * inv[a];
+        double t2 = (hi[a] - oo[a]) * inv[a];
         if (inv[a] < 0.0) { const double s = t1; t1 = t2; t2 = s; }
         tmin = smax(tmin, t1);
         tmax = smin(tmax, t2);
@@ -868,6 +871,9 @@ __device__ __forceinline__ bool box_static(const V3& o, const double inv[3], con
     }
     tmin_out = tmin;
     return tmax >= smax(tmin, 0.0);
+}
+__device__ __forceinline__ bool box_static(const V3& o, const double inv[3], const BvhNode& nd, double& tmin_out) {
+    return box_static6(o, inv, nd.lo, nd.hi, tmin_out);
 }
 
 // Traversal stack entry: idx < 0 marks a node whose box test already failed;

@@ -631,12 +631,11 @@ void build_bvh4(const std::vector<BvhNode>& bvh, const std::vector<int32_t>& roo
                 const BvhNode& nd = bvh[k];
                 for (int a = 0; a < 3; ++a) { q.lo[a][s] = round_down(nd.lo[a] - w); q.hi[a][s] = round_up(nd.hi[a] + w); }
                 if (nd.right >= 0) { q.ref[s] = k; todo.push_back(k); }
-                else q.ref[s] = -2 - k;
+                else q.ref[s] = -2 - ((nd.start << 3) | nd.count);
             };
             const BvhNode& B = bvh[b];
             if (B.right < 0) {          // a leaf root: one slot
                 put(0, b);
-                q.ref[0] = -2 - b;
                 continue;
             }
             const int32_t kids[2] = { b + 1, B.right };

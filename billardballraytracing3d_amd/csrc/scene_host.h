@@ -84,7 +84,8 @@ struct BvhNode {
 // 128 bytes: [axis][slot] lo and hi, then the slots' binary node indices.
 struct Bvh4Node {
     float lo[3][4], hi[3][4];
-    int32_t ref[4];   // >= 0: internal binary node (a 4-wide node), <= -2: leaf binary node -2 - ref, -1: empty
+    int32_t ref[4];   // >= 0: internal binary node (a 4-wide node), <= -2: leaf, -2 - ref = start << 3 | count
+                      // (its objects leaf[start .. start + count)), -1: empty
     int32_t axes;     // split axes that fix the depth-first order: B | left child << 2 | right child << 4
     int32_t pad[3];
 };
