@@ -323,17 +323,26 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
             }
         }
 
-        // one leaf object: SurfacePatch::intersect and the best-hit update (BVH.hpp:101-109)
+        // leaf objects: SurfacePatch::intersect and the best-hit update (BVH.hpp:101-109).  q is always an
+        // entered object; stage 1 runs on the leaf's entered objects in order until one hits (the lane then
+        // parks for Newton) or the leaf ends, so misses cost no extra pass of the loop.
+        auto next_object = [&]() -> bool {
+            ++q;
+            const uint32_t rest = q < leaf_count ? (enter >> q) : 0u;
+            if (rest == 0) { c.candidates += leaf_count - q; cur = -1; phase = kPhTrav; return false; }
+            const int nq = q + __ffs(rest) - 1;
+            c.candidates += nq - q;
+            q = nq;
+            return true;
+        };
         bool have_h = false;
         PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
         if (r >= 0 && phase == kPhLeaf) {
-            obj = S.leaf[leaf_start + q];
-            have_h = true;
-            if (enter & (1u << q)) {
+            for (;;) {
+                obj = S.leaf[leaf_start + q];
                 sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
-                if (sd.have) { phase = kPhSeed; have_h = false; }   // parked until the wave runs Newton
-            } else {
-                c.candidates++;
+                if (sd.have) { phase = kPhSeed; break; }   // parked until the wave runs Newton
+                if (!next_object()) break;
             }
         }
         // Newton for the parked lanes, all together: once newton_batch lanes wait, or
@@ -366,12 +375,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         if (have_h) {
             if (h.status < 0) { err = -h.status; err_obj = obj; phase = kPhDone; }
             else if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
-            if (phase == kPhLeaf) {            // the next entered object, or back to the walk
-                ++q;
-                const uint32_t rest = q < leaf_count ? (enter >> q) : 0u;
-                if (rest == 0) { c.candidates += leaf_count - q; cur = -1; phase = kPhTrav; }
-                else { const int nq = q + __ffs(rest) - 1; c.candidates += nq - q; q = nq; }
-            }
+            if (phase == kPhLeaf) next_object();   // the next entered object, or back to the walk
         }
 
         if (r >= 0 && phase == kPhDone) {
