@@ -337,12 +337,40 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         };
         bool have_h = false;
         PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
-        if (r >= 0 && phase == kPhLeaf) {
+        const unsigned long long in_leaf = __ballot(r >= 0 && phase == kPhLeaf);
+        if (COMPACT || (int)__popcll(in_leaf) > kCoopLanes) {
+            if (r >= 0 && phase == kPhLeaf) {
+                for (;;) {
+                    obj = S.leaf[leaf_start + q];
+                    sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
+                    if (sd.have) { phase = kPhSeed; break; }   // parked until the wave runs Newton
+                    if (!next_object()) break;
+                }
+            }
+        } else if (!COMPACT && in_leaf) {
+            // few lanes left here: the whole wave runs each one's stage 1 in turn
             for (;;) {
-                obj = S.leaf[leaf_start + q];
-                sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
-                if (sd.have) { phase = kPhSeed; break; }   // parked until the wave runs Newton
-                if (!next_object()) break;
+                const unsigned long long need = __ballot(r >= 0 && phase == kPhLeaf);
+                if (!need) break;
+                const int ld = __ffsll((long long)need) - 1;
+                if (lane == ld) obj = S.leaf[leaf_start + q];
+                const int32_t kk = __shfl(obj, ld);
+                const V3 lo_ = { __shfl(o.x, ld), __shfl(o.y, ld), __shfl(o.z, ld) };
+                const V3 ld_ = { __shfl(d.x, ld), __shfl(d.y, ld), __shfl(d.z, ld) };
+                const Ray32 lr = { __shfl(r32.ox, ld), __shfl(r32.oy, ld), __shfl(r32.oz, ld),
+                                   __shfl(r32.ix, ld), __shfl(r32.iy, ld), __shfl(r32.iz, ld) };
+                const int lscn = __shfl(scn, ld);
+                const bool lcull = __shfl(cull ? 1 : 0, ld) != 0;
+                uint32_t n_cull = 0, n_mt = 0;
+                const Stage1 b1 = stage1_coop<COMPACT ? 1 : 0>(S, kk, lo_, ld_, lr, S.scenes[lscn].cull_widen,
+                                                               lcull && S.meta[kk].cull, n_cull, n_mt);
+                if (lane == ld) {
+                    c.candidates++;
+                    c.cull_tests += n_cull;
+                    c.mt_tests += n_mt;
+                    if (b1.have) { sd = seed_of(b1); phase = kPhSeed; }   // parked until the wave runs Newton
+                    else next_object();
+                }
             }
         }
         // Newton for the parked lanes, all together: once newton_batch lanes wait, or

@@ -16,6 +16,7 @@ namespace {
 constexpr int kStackCap = 48;
 constexpr int kStack4Cap = 64;                // 4-wide traversal stack (3 entries per level; depth <= 21 levels)
 constexpr double kPruneSlack = 1.000001;      // FP32 tmin prune: relative slack over the FP64 best_t
+constexpr int kCoopLanes = 8;                 // stored scenes, leaf phase: at most this many lanes in it -> wave-cooperative stage 1
 constexpr int kDefaultWavesPerEu = 3;         // persistent-kernel occupancy target (best of 2/3/4 on MI355X)
 constexpr int kDefaultWavesPerEuCompact = 2;  // compact scenes: the on-the-fly cell evaluation wants the registers
 inline int default_waves_per_eu(bool compact) { return compact ? kDefaultWavesPerEuCompact : kDefaultWavesPerEu; }
@@ -773,6 +774,8 @@ struct Seed {
     bool have;
 };
 
+__device__ __forceinline__ Seed seed_of(const Stage1& best);
+
 template <int CM>
 __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r32,
                                             bool cull, double widen, Counters& c) {
@@ -784,6 +787,141 @@ __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const 
     if (CM == 1 || (CM == 2 && S.compact)) stage1_compact(S, k, o, d, r32, widen, use_cull, best, c);
     else if (use_cull) stage1_culled(S, k, G, C, o, d, r32, best, c);
     else stage1_brute(G, C, o, d, best, c);
+    return seed_of(best);
+}
+
+// k-th (0-based) set bit of m, which has more than k bits set.
+__device__ __forceinline__ int nth_bit(unsigned long long m, int k) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const int mid = lo + step;
+        if ((int)__popcll(m & ((1ull << mid) - 1ull)) <= k) lo = mid;
+    }
+    return lo;
+}
+
+// Stage 1 of one (ray, patch) pair by the whole wave (every lane calls it with
+// the same arguments).  The pyramid is walked level by level, each level's
+// boxes spread over the lanes (supers, then the blocks of the hit supers 4
+// supers a pass, the tiles of the hit blocks 7 a pass, the cells of the hit
+// tiles 7 a pass), and each hit cell's Moller-Trumbore tests run on a lane of
+// its own.  The cells tested are the per-lane walk's, and the best hit is the
+// minimum of (t, key) over them, so the result does not depend on the order;
+// the work counts (returned) are the per-lane walk's too.  For a wave with one
+// or two lanes left in the leaf phase, a level costs one memory round trip
+// instead of one per hit box.
+template <int CM>
+__device__ Stage1 stage1_coop(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r, double widen,
+                              bool use_cull, uint32_t& n_cull, uint32_t& n_mt) {
+    const int lane = __lane_id();
+    const double* G = S.cxyz + (size_t)k * bbrt::kCoarseN * bbrt::kCoarseN * 3;
+    const double* C = S.cell + (size_t)k * bbrt::kCells * bbrt::kCells * 4;
+    Counters dummy = {};
+    Stage1 best{ 0.0, 0, false };
+    n_cull = 0;
+    n_mt = 0;
+    auto test_cell = [&](int i, int j) -> bool {   // false: the compact cell's exact box rejects the ray
+        if constexpr (CM == 1) {
+            const CellData cd = compact_cell_any(S, k, i, j);
+            if (use_cull && !compact_cell_box(cd, widen, r)) return false;
+            cell_tests_data(cd, i, j, o, d, best, dummy);
+        } else {
+            cell_tests(G, C, i, j, o, d, best, dummy);
+        }
+        return true;
+    };
+    if (!use_cull) {
+        for (int base = 0; base < bbrt::kCells * bbrt::kCells; base += 64) {
+            const int cell = base + lane;
+            if (cell < bbrt::kCells * bbrt::kCells) test_cell(cell / bbrt::kCells, cell % bbrt::kCells);
+        }
+        n_mt = 6u * bbrt::kCells * bbrt::kCells;
+    } else {
+        const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
+        const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
+        const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
+        const float* gc = S.box_cell + (size_t)k * bbrt::kCellFloats;
+        const bool h0 = lane < 9 && box32(gs + 6 * lane, r);
+        const unsigned long long ms = __ballot(h0);
+        n_cull += 9;
+        const int ns = (int)__popcll(ms);
+        for (int sb = 0; sb < ns; sb += 4) {
+            // blocks of hit supers sb .. sb+3: lane = 16 * super + block slot
+            const int sk = sb + (lane >> 4);
+            const int sq = nth_bit(ms, min(sk, ns - 1));
+            const int bq = lane & 15;
+            const int blk = (bbrt::kBlocksPerSuper * (sq / 3) + (bq >> 2)) * bbrt::kBlockN +
+                            bbrt::kBlocksPerSuper * (sq % 3) + (bq & 3);
+            const bool h1 = sk < ns && box32(gb + sq * bbrt::kG16 + 6 * bq, r);   // empty slots never pass
+            n_cull += 16u * (uint32_t)min(4, ns - sb);
+            const unsigned long long m1 = __ballot(h1);
+            const int nb = (int)__popcll(m1);
+            for (int tb = 0; tb < nb; tb += 7) {
+                // tiles of hit blocks tb .. tb+6: lane = 9 * block + tile slot
+                const int bk = tb + lane / 9;
+                const int b1 = __shfl(blk, nth_bit(m1, min(bk, nb - 1)));
+                const int tq = lane % 9;
+                const int bi = b1 / bbrt::kBlockN, bj = b1 % bbrt::kBlockN;
+                const int tile = (3 * bi + tq / 3) * bbrt::kTileN + 3 * bj + tq % 3;
+                const bool h2 = lane < 63 && bk < nb && box32(gt + b1 * bbrt::kG9 + 6 * tq, r);
+                n_cull += 9u * (uint32_t)min(7, nb - tb);
+                const unsigned long long m2 = __ballot(h2);
+                const int nt = (int)__popcll(m2);
+                for (int cb = 0; cb < nt; cb += 7) {
+                    // cells of hit tiles cb .. cb+6: lane = 9 * tile + cell slot
+                    const int tk = cb + lane / 9;
+                    const int t1 = __shfl(tile, nth_bit(m2, min(tk, nt - 1)));
+                    const int cq = lane % 9;
+                    const int ti = t1 / bbrt::kTileN, tj = t1 % bbrt::kTileN;
+                    const int cell = (3 * ti + cq / 3) * bbrt::kCells + 3 * tj + cq % 3;
+                    bool h3 = false;
+                    if (lane < 63 && tk < nt) {
+                        if constexpr (CM == 1) {
+                            // the quantized box of the cell, relative to its tile's box
+                            const int tslot = 3 * (ti % 3) + tj % 3;
+                            const float* tb6 = gt + ((ti / 3) * bbrt::kBlockN + tj / 3) * bbrt::kG9 + 6 * tslot;
+                            const uint8_t* qc = S.qbox_cell + (size_t)k * bbrt::kQCellPatchBytes +
+                                                (size_t)t1 * bbrt::kQCellBytes + 6 * cq;
+                            const double tlo[3] = { tb6[0], tb6[1], tb6[2] };
+                            const double step[3] = { ((double)tb6[3] - tlo[0]) / 255.0, ((double)tb6[4] - tlo[1]) / 255.0,
+                                                     ((double)tb6[5] - tlo[2]) / 255.0 };
+                            const float bx[6] = { round_down_f(tlo[0] + (double)qc[0] * step[0]),
+                                                  round_down_f(tlo[1] + (double)qc[1] * step[1]),
+                                                  round_down_f(tlo[2] + (double)qc[2] * step[2]),
+                                                  round_up_f(tlo[0] + (double)qc[3] * step[0]),
+                                                  round_up_f(tlo[1] + (double)qc[4] * step[1]),
+                                                  round_up_f(tlo[2] + (double)qc[5] * step[2]) };
+                            h3 = box32(bx, r);
+                        } else {
+                            h3 = box32(gc + t1 * bbrt::kG9 + 6 * cq, r);
+                        }
+                    }
+                    n_cull += 9u * (uint32_t)min(7, nt - cb);
+                    const unsigned long long m3 = __ballot(h3);
+                    const int nc = (int)__popcll(m3);
+                    // one hit cell per lane (nc <= 63)
+                    const int c1 = __shfl(cell, nth_bit(m3, min(lane, nc - 1)));
+                    bool ran = false;
+                    if (lane < nc) ran = test_cell(c1 / bbrt::kCells, c1 % bbrt::kCells);
+                    n_mt += 6u * (uint32_t)__popcll(__ballot(ran));
+                }
+            }
+        }
+    }
+    // the wave's best (t, key)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double t2 = __shfl_xor(best.t, off);
+        const int k2 = __shfl_xor(best.key, off);
+        const bool h2 = __shfl_xor(best.have ? 1 : 0, off) != 0;
+        if (h2 && (!best.have || t2 < best.t || (t2 == best.t && k2 < best.key))) { best.t = t2; best.key = k2; best.have = true; }
+    }
+    return best;
+}
+
+// The triangle-centroid seed of a stage-1 hit (SurfacePatch.hpp:298-301).
+__device__ __forceinline__ Seed seed_of(const Stage1& best) {
     Seed sd{ 0.0, 0.0, best.t, best.have };
     if (!best.have) return sd;
     const int cellk = best.key / 6;
