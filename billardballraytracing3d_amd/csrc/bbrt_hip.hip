@@ -338,7 +338,8 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         bool have_h = false;
         PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
         const unsigned long long in_leaf = __ballot(r >= 0 && phase == kPhLeaf);
-        if (COMPACT || (int)__popcll(in_leaf) > kCoopLanes) {
+        const int coop_max = (flags & BBRT_FLAG_STAGE1_WAVE) ? 64 : ((flags & BBRT_FLAG_STAGE1_LANE) ? 0 : kCoopLanes);
+        if (COMPACT || (int)__popcll(in_leaf) > coop_max) {
             if (r >= 0 && phase == kPhLeaf) {
                 for (;;) {
                     obj = S.leaf[leaf_start + q];

@@ -65,9 +65,12 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
        BBRT_FLAG_TIME_KERNELS = 4u,     /* record HIP events around every bounce-kernel launch */
        BBRT_FLAG_WAVEFRONT = 8u,        /* one launch per bounce with live-ray compaction instead of the
                                            persistent kernel with a device ray queue */
-       BBRT_FLAG_GRAPH = 16u };         /* capture the trace (counter fills, k_init, persistent kernel) in a
+       BBRT_FLAG_GRAPH = 16u,           /* capture the trace (counter fills, k_init, persistent kernel) in a
                                            hipGraph on the handle's stream and replay it while the arguments
                                            are unchanged; kernel_ms[0] is then the whole graph's time */
+       BBRT_FLAG_STAGE1_WAVE = 32u,     /* persistent kernel, stored scenes: every stage 1 wave-cooperative */
+       BBRT_FLAG_STAGE1_LANE = 64u };   /* ... every stage 1 per lane (default: wave-cooperative when at most
+                                           8 lanes of a wave need one); results and counters do not change */
 
 typedef struct {
     int32_t newton_mode;   /* BBRT_NEWTON_* */

@@ -95,6 +95,23 @@ def test_wavefront_schedule_matches_persistent(name):
 
 
 @pytest.mark.parametrize("name", ["c1", "c2s"])
+def test_wave_cooperative_stage1_matches_per_lane(name):
+    """Stage 1 run by the whole wave for every candidate, or by each lane alone, gives the default run's
+    records and work counters."""
+    sc, o, d, mb = _scene(name)
+    gs = _gpu_scene(name, sc)
+    print(f"[stage1 wave/lane] scene={name}", flush=True)
+    a = gs.trace(o, d, mb, mode=L.NEWTON_CONVERGED)
+    for fl in (L.FLAG_STAGE1_WAVE, L.FLAG_STAGE1_LANE, L.FLAG_STAGE1_WAVE | L.FLAG_NO_CULL):
+        b = gs.trace(o, d, mb, mode=L.NEWTON_CONVERGED, flags=fl)
+        assert_same(a, b, f"{name} flags {fl}")
+        keys = ("bounces", "candidates", "newton_calls", "newton_iters") + \
+            (("cull_box_tests", "mt_tests", "bvh_node_tests") if not fl & L.FLAG_NO_CULL else ())
+        for k in keys:
+            assert a.counter(k) == b.counter(k), (fl, k)
+
+
+@pytest.mark.parametrize("name", ["c1", "c2s"])
 def test_culling_equals_brute_force(name):
     sc, o, d, mb = _scene(name)
     gs = _gpu_scene(name, sc)
