@@ -166,6 +166,8 @@ def main():
                      "flops_per_launch": flops / launches if launches else None},
         "mean_newton_iters": {"triangle": newton_iters / newton_calls if newton_calls else None, "learned": None},
         "bounces_per_step": bounces // args.steps,
+        "rays_by_status": status_counts(outs["status"]),
+        "errors_by_kind": error_counts(outs["err_kind"]),
         "counters_per_step": per_step,
         "scene_build_s": t_build,
     }
@@ -261,6 +263,22 @@ def batch_leg(args):
         "scene_generation_s": t1 - t0, "scene_build_s": t2 - t1,
         "counters_per_step": {k: int(ctr[v] // args.batch_steps) for k, v in L.CTR.items()},
     }
+
+
+ERROR_KINDS = ("none", "newton_singular", "newton_noconv", "newton_domain", "backface", "zero_dir", "stack")
+
+
+def error_counts(err_kind):
+    """Rays of the last step per error kind (device int8 array)."""
+    import torch
+    c = torch.bincount(err_kind.to(torch.int64).clamp(0, len(ERROR_KINDS) - 1), minlength=len(ERROR_KINDS)).cpu()
+    return {k: int(v) for k, v in zip(ERROR_KINDS, c.tolist()) if k != "none"}
+
+
+def status_counts(status):
+    import torch
+    c = torch.bincount(status.to(torch.int64).clamp(0, 2), minlength=3).cpu().tolist()
+    return {"escaped": int(c[0]), "trapped": int(c[1]), "error": int(c[2])}
 
 
 def _progress(msg):
@@ -369,6 +387,7 @@ def c5_leg(args):
                                     "mean": float(per.mean()), "max": float(per.max())} if per.size else None,
         "oracle_check": {"scenes": 6, "rays": rays_checked, "rays_with_mismatch": mism, "max_rel_hit_error": max_err,
                          "oracle": "CPU FP64, triangle seed, converged Newton"},
+        "rays_by_status": status_counts(outs["status"]), "errors_by_kind": error_counts(outs["err_kind"]),
         "bounces_per_step": bounces // steps, "train": {"pairs": rep["n_pairs"], "steps": rep["steps"],
                                                          "test_loss": rep["test_loss"]},
         "scene_generation_s": t1 - t0, "scene_build_s": t2 - t1, "train_s": t3 - t2,
