@@ -149,6 +149,27 @@ def test_edge_cases_match_oracle():
     assert e.status.size == 0 and e.counter("rays") == 0
 
 
+def test_far_origins_match_oracle():
+    """Rays starting beyond the culling limit (8 x the scene scale): the first bounce walks the BVH with FP64 slot
+    tests and runs brute-force stage 1; later bounces start inside the scene and use the FP32 walk."""
+    from oracle import oracle as O
+    sc, o, d, mb = _scene("c1")
+    rng = np.random.default_rng(5)
+    n = 96
+    target = rng.uniform(-2.0, 2.0, (n, 3)) * [1.0, 0.5, 0.5]
+    dirn = rng.normal(size=(n, 3))
+    dirn /= np.linalg.norm(dirn, axis=1, keepdims=True)
+    oo = target - 60.0 * dirn                          # 60 units out, aimed at the spheres
+    gs = _gpu_scene("c1", sc)
+    print(f"[far] rays={n}", flush=True)
+    for mode in (L.NEWTON_REF, L.NEWTON_CONVERGED):
+        g = gs.trace(oo, dirn, mb, mode=mode)
+        r = O.OracleScene(sc).trace(oo, dirn, mb, mode=mode)
+        assert_same(g, r, f"far origins mode{mode}")
+        assert g.counter("bounces") > n // 4
+        assert g.counter("candidates") == r.counters[2]
+
+
 def test_hit_counts_on_device():
     """PlaneSourceDriver semantics (PlaneSourceDriver.hpp:96-108) accumulated on the GPU."""
     sc, o, d, mb = _scene("c1")
