@@ -53,11 +53,11 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-no-tune", action="store_true", help="batched leg: library default launch shape")
     ap.add_argument("--batch-only", action="store_true", help="run the batched-scenes leg alone")
-    ap.add_argument("--workload", choices=["sinai", "c5"], default="sinai",
-                    help="c5: the full batched form (512 scenes, 16M rays, 200 bounces, learned seeds) alone")
+    ap.add_argument("--workload", choices=["sinai", "full-form"], default="sinai",
+                    help="full-form: the full batched form (512 scenes, 16M rays, 200 bounces, learned seeds) alone")
     ap.add_argument("--no-full-form", action="store_true", help="skip the full batched form leg (512 scenes, 16M rays)")
-    ap.add_argument("--c5-scenes", type=int, default=512)
-    ap.add_argument("--c5-rays", type=int, default=16_000_000)
+    ap.add_argument("--full-scenes", type=int, default=512)
+    ap.add_argument("--full-rays", type=int, default=16_000_000)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -77,8 +77,8 @@ def main():
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
-    if args.workload == "c5":
-        r = c5_leg(args)
+    if args.workload == "full-form":
+        r = full_form_leg(args)
         print(json.dumps(r), flush=True)
         return
     if args.batch_only:
@@ -89,7 +89,7 @@ def main():
                           "data": "synthetic", "config": {"workload": b["workload"]}, "batch": b}), flush=True)
         return
     t0 = time.time()
-    sc, o, d, mb = S.config2(seed=args.seed, n_rays=args.rays)
+    sc, o, d, mb = S.sinai_cone(seed=args.seed, n_rays=args.rays)
     mb = args.max_bounces
     K = mb if args.record_k < 0 else args.record_k
     gs = T.GpuScene(sc)
@@ -195,7 +195,7 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_full_form:
         _progress("full batched form leg")
-        res["full_form"] = c5_leg(args)
+        res["full_form"] = full_form_leg(args)
     print(json.dumps(res), flush=True)
 
 
@@ -285,8 +285,8 @@ def _progress(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def c5_leg(args):
-    """The full batched form: args.c5_scenes scenes of the batched layout, args.c5_rays rays in total, 200
+def full_form_leg(args):
+    """The full batched form: args.full_scenes scenes of the batched layout, args.full_rays rays in total, 200
     bounces, learned Newton seeds (network trained on the GPU on pairs of the Sinai scene) with the
     triangle-seed fallback, one hipGraph-replayed persistent launch with an autotuned shape.  Reports the
     fallback rate per scene and checks sampled scenes against the CPU oracle (triangle seed, converged)."""
@@ -297,12 +297,12 @@ def c5_leg(args):
 
     dev = torch.device("cuda", 0)
     t0 = time.time()
-    scenes, o, d, rb = S.batch_config(args.seed, n_scenes=args.c5_scenes, total_rays=args.c5_rays)
+    scenes, o, d, rb = S.batch_config(args.seed, n_scenes=args.full_scenes, total_rays=args.full_rays)
     t1 = time.time()
-    _progress(f"c5: {len(scenes)} scenes, {int(rb[-1])} rays generated in {t1 - t0:.1f} s")
+    _progress(f"full form: {len(scenes)} scenes, {int(rb[-1])} rays generated in {t1 - t0:.1f} s")
     gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
     t2 = time.time()
-    _progress(f"c5: {gb.n_patches} patches built (compact, device build) in {t2 - t1:.1f} s")
+    _progress(f"full form: {gb.n_patches} patches built (compact, device build) in {t2 - t1:.1f} s")
     # the network: trained on the Sinai scene (a compact handle of its own), then attached to the batch
     sinai = T.GpuScene.batch([S.sinai_scene(0)], compact=True, device_build=True)
     net = T.Mlp(args.seed + 3)
@@ -311,7 +311,7 @@ def c5_leg(args):
     del sinai
     gb.set_mlp(net)
     t3 = time.time()
-    _progress(f"c5: network trained in {t3 - t2:.1f} s, test loss {rep['test_loss']:.3g}")
+    _progress(f"full form: network trained in {t3 - t2:.1f} s, test loss {rep['test_loss']:.3g}")
     n = int(rb[-1])
     mb, K = 200, 8
     to = torch.from_numpy(o).to(dev)
@@ -326,7 +326,7 @@ def c5_leg(args):
     st = torch.cuda.current_stream(dev).cuda_stream
     best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
                              seed=L.SEED_LEARNED)
-    _progress(f"c5: autotuned {best} over {len(rows)} shapes")
+    _progress(f"full form: autotuned {best} over {len(rows)} shapes")
 
     def step():
         return gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
@@ -344,7 +344,7 @@ def c5_leg(args):
     torch.cuda.synchronize()
     te = time.perf_counter()
     bounces = int(ctr[L.CTR["bounces"]])
-    _progress(f"c5: {steps} steps, {(te - ts) * 1e3 / steps:.1f} ms/step, {bounces / (te - ts):.4g} bounces/s")
+    _progress(f"full form: {steps} steps, {(te - ts) * 1e3 / steps:.1f} ms/step, {bounces / (te - ts):.4g} bounces/s")
     # fallback rate per scene
     ss = outs["seed_stats"].cpu().numpy().astype(np.int64)
     per = []
@@ -371,7 +371,7 @@ def c5_leg(args):
         scale = np.maximum(np.linalg.norm(r.hit_xyz, axis=-1), 1.0)
         err = np.where(m, np.linalg.norm(hx[sl] - r.hit_xyz, axis=-1) / scale, 0.0)
         max_err = max(max_err, float(err.max()))
-        _progress(f"c5: oracle check scene {i}: {r.status.size} rays, {mism} mismatches so far")
+        _progress(f"full form: oracle check scene {i}: {r.status.size} rays, {mism} mismatches so far")
     hits = int(ctr[L.CTR["seed_learned"]] + ctr[L.CTR["seed_fallback"]])
     return {
         "metric": METRIC, "value": bounces / (te - ts), "unit": "bounces/s", "n_gpus": 1, "steps": steps,

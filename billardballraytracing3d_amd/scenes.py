@@ -418,7 +418,7 @@ def cone_rays(n: int, seed: int, half_extent: float, z0: float, cone_full_deg: f
     return np.ascontiguousarray(o), np.ascontiguousarray(d)
 
 
-def config1(seed: int = 0, nx: int = 64, ny: int = 64):
+def two_spheres_plane_source(seed: int = 0, nx: int = 64, ny: int = 64):
     """Two P2 unit spheres + 64x64 plane source at z=-5 along +z; 20 bounces."""
     sc = two_spheres_scene(seed)
     o = plane_grid((-3.0, -3.0, -5.0), (3.0, 3.0, -5.0), nx, ny)
@@ -426,7 +426,7 @@ def config1(seed: int = 0, nx: int = 64, ny: int = 64):
     return sc, o, np.ascontiguousarray(d), 20
 
 
-def config2(seed: int = 0, n_rays: int = 1 << 20):
+def sinai_cone(seed: int = 0, n_rays: int = 1 << 20):
     """Sinai-like P4 scene (10,240 patches) + n_rays in a 10 degree cone from z=-8; 100 bounces."""
     sc = sinai_scene(seed)
     o, d = cone_rays(n_rays, seed, 6.0, -8.0, 10.0)

@@ -23,8 +23,8 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("mode", [L.NEWTON_REF, L.NEWTON_CONVERGED])
-def test_compact_equals_stored_config1(mode):
-    sc, o, d, mb = S.config1()
+def test_compact_equals_stored_two_spheres(mode):
+    sc, o, d, mb = S.two_spheres_plane_source()
     st = T.GpuScene(sc)
     cp = T.GpuScene.batch([sc], compact=True)
     _same(cp.trace(o, d, mb, mode=mode), st.trace(o, d, mb, mode=mode))
@@ -32,7 +32,7 @@ def test_compact_equals_stored_config1(mode):
 
 
 def test_compact_equals_stored_sinai_sample():
-    sc, o, d, mb = S.config2(0, 1 << 20)
+    sc, o, d, mb = S.sinai_cone(0, 1 << 20)
     sel = slice(0, 1 << 20, 512)
     st = T.GpuScene(sc)
     cp = T.GpuScene.batch([sc], compact=True)
@@ -49,14 +49,14 @@ def test_compact_batch_equals_stored_batch():
 
 
 def test_compact_brute_force_equals_stored():
-    sc, o, d, mb = S.config1(nx=8, ny=8)
+    sc, o, d, mb = S.two_spheres_plane_source(nx=8, ny=8)
     st = T.GpuScene(sc)
     cp = T.GpuScene.batch([sc], compact=True)
     _same(cp.trace(o, d, 3, flags=L.FLAG_NO_CULL), st.trace(o, d, 3, flags=L.FLAG_NO_CULL))
 
 
 def test_compact_scene_has_no_coarse_grid():
-    sc, _, _, _ = S.config1(nx=2, ny=2)
+    sc, _, _, _ = S.two_spheres_plane_source(nx=2, ny=2)
     cp = T.GpuScene.batch([sc], compact=True)
     with pytest.raises(L.BBRTError, match="compact"):
         cp.coarse(0)
@@ -73,5 +73,5 @@ def test_device_build_equals_host_build():
     for key in ba:
         np.testing.assert_array_equal(ba[key], bb[key], err_msg=key)
     _same(a.trace(o, d, 25, ray_begin=rb), b.trace(o, d, 25, ray_begin=rb))
-    sc, o1, d1, mb = S.config1()
+    sc, o1, d1, mb = S.two_spheres_plane_source()
     _same(T.GpuScene.batch([sc], compact=True, device_build=True).trace(o1, d1, mb), T.GpuScene(sc).trace(o1, d1, mb))

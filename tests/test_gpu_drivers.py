@@ -33,8 +33,8 @@ def load(path, dtype):
 
 @pytest.fixture(scope="module")
 def c1dir(tmp_path_factory):
-    sc, o, d, mb = S.config1()
-    d_ = tmp_path_factory.mktemp("c1")
+    sc, o, d, mb = S.two_spheres_plane_source()
+    d_ = tmp_path_factory.mktemp("two_spheres")
     S.write_patch_dir(sc, str(d_))
     return sc, o, d, mb, str(d_) + "/"
 

@@ -72,7 +72,7 @@ def test_gradient_matches_oracle():
 
 @pytest.fixture(scope="module")
 def c1():
-    sc, o, d, mb = S.config1()
+    sc, o, d, mb = S.two_spheres_plane_source()
     return sc, T.GpuScene(sc), O.OracleScene(sc), o, d, mb
 
 

@@ -21,10 +21,10 @@ _CACHE = {}
 
 def _scene(name):
     if name not in _CACHE:
-        if name == "c1":
-            sc, o, d, mb = S.config1()
-        elif name == "c2s":
-            sc, o, d, mb = S.config2(n_rays=2048)
+        if name == "two_spheres":
+            sc, o, d, mb = S.two_spheres_plane_source()
+        elif name == "sinai_sample":
+            sc, o, d, mb = S.sinai_cone(n_rays=2048)
         else:
             raise KeyError(name)
         _CACHE[name] = (sc, o, d, mb)
@@ -58,7 +58,7 @@ def assert_same(g, r, label):
     np.testing.assert_array_equal(g.final_od, r.final_od, err_msg=label + " final origin/direction")
 
 
-@pytest.mark.parametrize("name", ["c1", "c2s"])
+@pytest.mark.parametrize("name", ["two_spheres", "sinai_sample"])
 @pytest.mark.parametrize("mode", [L.NEWTON_REF, L.NEWTON_CONVERGED])
 def test_trace_matches_oracle(name, mode):
     sc, o, d, mb = _scene(name)
@@ -81,7 +81,7 @@ def test_trace_matches_oracle(name, mode):
     assert g.counter("rays") == o.shape[0]
 
 
-@pytest.mark.parametrize("name", ["c1", "c2s"])
+@pytest.mark.parametrize("name", ["two_spheres", "sinai_sample"])
 def test_wavefront_schedule_matches_persistent(name):
     sc, o, d, mb = _scene(name)
     gs = _gpu_scene(name, sc)
@@ -94,7 +94,7 @@ def test_wavefront_schedule_matches_persistent(name):
     assert a.counter("bvh_node_tests") < b.counter("bvh_node_tests")   # 4-wide walk vs the binary one
 
 
-@pytest.mark.parametrize("name", ["c1", "c2s"])
+@pytest.mark.parametrize("name", ["two_spheres", "sinai_sample"])
 def test_wave_cooperative_stage1_matches_per_lane(name):
     """Stage 1 run by the whole wave for every candidate, or by each lane alone, gives the default run's
     records and work counters."""
@@ -111,7 +111,7 @@ def test_wave_cooperative_stage1_matches_per_lane(name):
             assert a.counter(k) == b.counter(k), (fl, k)
 
 
-@pytest.mark.parametrize("name", ["c1", "c2s"])
+@pytest.mark.parametrize("name", ["two_spheres", "sinai_sample"])
 def test_culling_equals_brute_force(name):
     sc, o, d, mb = _scene(name)
     gs = _gpu_scene(name, sc)
@@ -128,7 +128,7 @@ def test_edge_cases_match_oracle():
     rays, max_bounces 0 and 1, a ragged ray count, records shorter than the
     bounce cap, and an empty batch."""
     from oracle import oracle as O
-    sc, o, d, mb = _scene("c1")
+    sc, o, d, mb = _scene("two_spheres")
     rng = np.random.default_rng(3)
     n = 333
     oo = rng.uniform(-3, 3, (n, 3))
@@ -136,7 +136,7 @@ def test_edge_cases_match_oracle():
     dd[::11] = 0.0                                    # zero-length: Error, zero_dir
     oo[1::13] = [-1.5, 0.1, 0.2]                      # inside the left sphere: back-face hits
     dd[2::17] = [[1.0, 0.0, 0.0]]                     # axis-parallel: 0 * inf in the slab test
-    gs = _gpu_scene("c1", sc)
+    gs = _gpu_scene("two_spheres", sc)
     osc = O.OracleScene(sc)
     for max_b, rec in [(0, 1), (1, 1), (20, 20), (20, 5)]:
         print(f"[edge] max_bounces={max_b} record_k={rec}", flush=True)
@@ -153,14 +153,14 @@ def test_far_origins_match_oracle():
     """Rays starting beyond the culling limit (8 x the scene scale): the first bounce walks the BVH with FP64 slot
     tests and runs brute-force stage 1; later bounces start inside the scene and use the FP32 walk."""
     from oracle import oracle as O
-    sc, o, d, mb = _scene("c1")
+    sc, o, d, mb = _scene("two_spheres")
     rng = np.random.default_rng(5)
     n = 96
     target = rng.uniform(-2.0, 2.0, (n, 3)) * [1.0, 0.5, 0.5]
     dirn = rng.normal(size=(n, 3))
     dirn /= np.linalg.norm(dirn, axis=1, keepdims=True)
     oo = target - 60.0 * dirn                          # 60 units out, aimed at the spheres
-    gs = _gpu_scene("c1", sc)
+    gs = _gpu_scene("two_spheres", sc)
     print(f"[far] rays={n}", flush=True)
     for mode in (L.NEWTON_REF, L.NEWTON_CONVERGED):
         g = gs.trace(oo, dirn, mb, mode=mode)
@@ -172,8 +172,8 @@ def test_far_origins_match_oracle():
 
 def test_hit_counts_on_device():
     """PlaneSourceDriver semantics (PlaneSourceDriver.hpp:96-108) accumulated on the GPU."""
-    sc, o, d, mb = _scene("c1")
-    gs = _gpu_scene("c1", sc)
+    sc, o, d, mb = _scene("two_spheres")
+    gs = _gpu_scene("two_spheres", sc)
     for direct in (False, True):
         g = gs.trace(o, d, mb, mode=L.NEWTON_REF, hit_count=True, flags=L.FLAG_COUNT_DIRECT_HIT if direct else 0)
         want = np.zeros(sc.n_patches, np.int64)
