@@ -79,9 +79,8 @@ typedef struct {
     int32_t record_k;      /* bounces recorded per ray in hit_patch / hit_xyz / newton_iters */
     uint32_t flags;        /* BBRT_FLAG_* */
     int32_t block;         /* threads per block of the bounce kernel; 0 = library default */
-    int32_t waves_per_eu;  /* occupancy target of the persistent kernel: 0 = library default (3; 2 on compact
-                              scenes), or 2, 3, 4 (the compiler keeps registers within that occupancy,
-                              spilling if needed) */
+    int32_t waves_per_eu;  /* occupancy target of the persistent kernel: 0 = library default (2), or 2, 3, 4
+                              (the compiler keeps registers within that occupancy, spilling if needed) */
     float grid_fraction;   /* persistent kernel: fraction of the resident workgroups launched (0 = all) */
     int32_t newton_batch;  /* persistent kernel: lanes of a wave whose stage 1 hit wait until this many can run
                               Newton (and the learned seed's network) together (1..64; 0 = library default, 64);
