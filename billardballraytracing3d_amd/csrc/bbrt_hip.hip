@@ -118,12 +118,13 @@ __global__ void __launch_bounds__(1024) k_init(const double* __restrict__ origin
 
 // Persistent bounce loop as a per-lane state machine under a wave-uniform
 // outer loop.  Each iteration: lanes without a ray take one from the live list
-// (one wave-aggregated atomic); traversing lanes walk internal nodes until they
-// stand on a leaf (while-while); lanes on a leaf intersect one of its objects;
-// lanes whose bounce is complete record it, reflect, and start the next
-// bounce at once -- a lane never waits for another lane's whole bounce.  Every
-// ray ends within max_bounces bounces and the list is finite, so every wave
-// leaves the loop.
+// (one wave-aggregated atomic); traversing lanes walk 4-wide nodes until they
+// stand on a leaf that passes its exact test (while-while); lanes on a leaf run
+// stage 1 on its entered objects until one hits (the whole wave runs each
+// lane's stage 1 in turn when few lanes are in this phase); lanes whose bounce
+// is complete record it, reflect, and start the next bounce at once -- a lane
+// never waits for another lane's whole bounce.  Every ray ends within
+// max_bounces bounces and the list is finite, so every wave leaves the loop.
 enum : int { kPhTrav = 1, kPhLeaf = 2, kPhDone = 3, kPhSeed = 4 };
 
 // A lane whose stage 1 hits parks (kPhSeed) with its triangle seed; once
@@ -160,7 +161,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
     Ray32 r32 = { 0, 0, 0, 0, 0, 0 };
     Ent4 stack[kStack4Cap];
     int sp = 0;
-    int32_t cur = -1;                         // 4-wide node (>= 0), leaf (-2 - binary node) or nothing (-1)
+    int32_t cur = -1;                         // 4-wide node (>= 0), leaf (-2 - (start << 3 | count)) or nothing (-1)
     float cur_t = 0.0f;                       // its entry t (FP32, prune only)
     uint32_t negm = 0;                        // direction sign bits: the depth-first order
     double best_t = INFINITY;
@@ -171,7 +172,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
     uint32_t enter = 0;
     int32_t obj = 0;                          // the leaf object being intersected
     uint32_t sl0 = 0, sf0 = 0;                // learned mode: seed counters when the ray started
-    Seed sd{ 0.0, 0.0, 0.0, false };          // its stage-1 seed (learned mode, while parked)
+    Seed sd{ 0.0, 0.0, 0.0, false };          // its stage-1 seed, while parked for Newton
 
     // bounce set-up: RayTracingDriver.hpp:187 and the root of BVH.hpp:89
     auto start_bounce = [&](bool root_live) {
