@@ -19,6 +19,8 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
@@ -166,6 +168,8 @@ public:
     };
 
     explicit SurfacePatch(const std::string& path) : data_(load_patch_file(path)) { compute_centroid_and_box(); }
+    // From values already parsed (the binary patch cache).
+    explicit SurfacePatch(Data d) : data_(std::move(d)) { compute_centroid_and_box(); }
     SurfacePatch(const std::string& directory, const std::string& file_prefix, long long index)
         : SurfacePatch(directory + file_prefix + std::to_string(index + 1) + ".txt") {}
 
@@ -210,10 +214,6 @@ public:
         max_sagitta_ = max_sag;
     }
 
-private:
-    Data data_;
-    double max_sagitta_ = 0.0;
-
     // SurfacePatch.hpp:313-371
     static Data load_patch_file(const std::string& path) {
         std::ifstream fin(path);
@@ -246,6 +246,10 @@ private:
         return d;
     }
 
+private:
+    Data data_;
+    double max_sagitta_ = 0.0;
+
     // SurfacePatch.hpp:380-407 (the box is replaced by the inflated one once the scene is built)
     void compute_centroid_and_box() {
         const long long n = data_.point_count();
@@ -267,6 +271,170 @@ private:
 };
 
 // --- RayTracingDriver (RayTracingDriver.hpp:65-425) --------------------------
+// --- Binary patch cache -------------------------------------------------------
+// Parsing a large IFGF-RP directory as text dominates a driver's start-up.  The
+// cache keeps every parsed SurfacePatch::Data of a file list in one binary file,
+// keyed by the files' paths, sizes and modification times; any change to the
+// list or to a file makes it stale, and it is then rebuilt from the text.  The
+// values are stored bit for bit, so a scene built from the cache is the scene
+// built from the text.  Off unless patch_cache_directory() is set.
+namespace detail {
+inline std::string& patch_cache_dir() {
+    static std::string dir;
+    return dir;
+}
+
+struct PatchFileKey {
+    std::string path;
+    uint64_t size = 0;
+    int64_t mtime = 0;
+};
+
+inline std::vector<PatchFileKey> patch_file_keys(const std::vector<std::string>& files) {
+    namespace fs = std::filesystem;
+    std::vector<PatchFileKey> keys;
+    for (const auto& f : files) {
+        PatchFileKey k;
+        k.path = fs::absolute(f).lexically_normal().string();
+        k.size = (uint64_t)fs::file_size(f);
+        k.mtime = (int64_t)fs::last_write_time(f).time_since_epoch().count();
+        keys.push_back(std::move(k));
+    }
+    return keys;
+}
+
+// The cache file of a file list: FNV-1a of the normalised paths, in the cache directory.
+inline std::string patch_cache_file(const std::vector<PatchFileKey>& keys) {
+    uint64_t h = 1469598103934665603ull;
+    for (const auto& k : keys) {
+        for (unsigned char ch : k.path) { h ^= ch; h *= 1099511628211ull; }
+        h ^= 0xFF; h *= 1099511628211ull;
+    }
+    char name[40];
+    std::snprintf(name, sizeof(name), "bbrt_patches_%016llx.bin", (unsigned long long)h);
+    return (std::filesystem::path(patch_cache_dir()) / name).string();
+}
+
+constexpr char kPatchCacheMagic[8] = { 'B', 'B', 'R', 'T', 'P', 'C', '1', '\0' };
+
+template <class T>
+inline void put_vec(std::ofstream& f, const std::vector<T>& v) {
+    const uint64_t n = v.size();
+    f.write(reinterpret_cast<const char*>(&n), sizeof(n));
+    f.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(n * sizeof(T)));
+}
+template <class T>
+inline bool get_vec(std::ifstream& f, std::vector<T>& v) {
+    uint64_t n = 0;
+    if (!f.read(reinterpret_cast<char*>(&n), sizeof(n)) || n > (1ull << 40) / sizeof(T)) return false;
+    v.resize(n);
+    return (bool)f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(T)));
+}
+
+inline void write_patch_cache(const std::string& file, const std::vector<PatchFileKey>& keys,
+                              const std::vector<const SurfacePatch::Data*>& data) {
+    const std::string tmp = file + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+        if (!f) return;   // a cache that cannot be written is skipped, not an error
+        f.write(kPatchCacheMagic, 8);
+        const uint64_t n = keys.size();
+        f.write(reinterpret_cast<const char*>(&n), sizeof(n));
+        for (const auto& k : keys) {
+            const uint64_t len = k.path.size();
+            f.write(reinterpret_cast<const char*>(&len), sizeof(len));
+            f.write(k.path.data(), (std::streamsize)len);
+            f.write(reinterpret_cast<const char*>(&k.size), sizeof(k.size));
+            f.write(reinterpret_cast<const char*>(&k.mtime), sizeof(k.mtime));
+        }
+        for (const SurfacePatch::Data* d : data) {
+            const int32_t dims[3] = { d->imax, d->jmax, d->kmax };
+            f.write(reinterpret_cast<const char*>(dims), sizeof(dims));
+            f.write(reinterpret_cast<const char*>(&d->zoneID), sizeof(d->zoneID));
+            for (const auto* v : { &d->uNodes, &d->vNodes, &d->uWeights, &d->vWeights, &d->x, &d->y, &d->z })
+                put_vec(f, *v);
+            put_vec(f, d->mask);
+            for (const auto* v : { &d->dxdu, &d->dydu, &d->dzdu, &d->dxdv, &d->dydv, &d->dzdv, &d->dS, &d->nuX,
+                                   &d->nuY, &d->nuZ })
+                put_vec(f, *v);
+            f.write(reinterpret_cast<const char*>(d->edge_flags.data()), sizeof(d->edge_flags));
+        }
+        if (!f) return;
+    }
+    std::error_code ec;
+    std::filesystem::rename(tmp, file, ec);
+}
+
+// True and the parsed patches when the cache file matches the keys exactly.
+inline bool read_patch_cache(const std::string& file, const std::vector<PatchFileKey>& keys,
+                             std::vector<SurfacePatch::Data>& out) {
+    std::ifstream f(file, std::ios::binary);
+    if (!f) return false;
+    char magic[8];
+    if (!f.read(magic, 8) || std::memcmp(magic, kPatchCacheMagic, 8) != 0) return false;
+    uint64_t n = 0;
+    if (!f.read(reinterpret_cast<char*>(&n), sizeof(n)) || n != keys.size()) return false;
+    for (const auto& k : keys) {
+        uint64_t len = 0;
+        if (!f.read(reinterpret_cast<char*>(&len), sizeof(len)) || len != k.path.size()) return false;
+        std::string path(len, '\0');
+        uint64_t size = 0;
+        int64_t mtime = 0;
+        if (!f.read(&path[0], (std::streamsize)len) || path != k.path) return false;
+        if (!f.read(reinterpret_cast<char*>(&size), sizeof(size)) || size != k.size) return false;
+        if (!f.read(reinterpret_cast<char*>(&mtime), sizeof(mtime)) || mtime != k.mtime) return false;
+    }
+    std::vector<SurfacePatch::Data> data(n);
+    for (auto& d : data) {
+        int32_t dims[3];
+        if (!f.read(reinterpret_cast<char*>(dims), sizeof(dims))) return false;
+        d.imax = dims[0]; d.jmax = dims[1]; d.kmax = dims[2];
+        if (!f.read(reinterpret_cast<char*>(&d.zoneID), sizeof(d.zoneID))) return false;
+        for (auto* v : { &d.uNodes, &d.vNodes, &d.uWeights, &d.vWeights, &d.x, &d.y, &d.z })
+            if (!get_vec(f, *v)) return false;
+        if (!get_vec(f, d.mask)) return false;
+        for (auto* v : { &d.dxdu, &d.dydu, &d.dzdu, &d.dxdv, &d.dydv, &d.dzdv, &d.dS, &d.nuX, &d.nuY, &d.nuZ })
+            if (!get_vec(f, *v)) return false;
+        if (!f.read(reinterpret_cast<char*>(d.edge_flags.data()), sizeof(d.edge_flags))) return false;
+        if (d.x.size() != (size_t)d.point_count()) return false;
+    }
+    out = std::move(data);
+    return true;
+}
+
+// The patches of a file list: from the cache when it is current, else parsed
+// from the text (and the cache rewritten).  from_cache tells which.
+inline std::vector<std::unique_ptr<SurfacePatch>> load_patch_list(const std::vector<std::string>& files,
+                                                                  bool* from_cache = nullptr) {
+    std::vector<std::unique_ptr<SurfacePatch>> patches;
+    if (from_cache) *from_cache = false;
+    bool all_there = true;
+    for (const auto& f : files) all_there = all_there && std::filesystem::is_regular_file(f);
+    if (patch_cache_dir().empty() || !all_there) {   // (a missing file fails in the parser, as without a cache)
+        for (const auto& f : files) patches.push_back(std::make_unique<SurfacePatch>(f));
+        return patches;
+    }
+    const auto keys = patch_file_keys(files);
+    const std::string cache = patch_cache_file(keys);
+    std::vector<SurfacePatch::Data> data;
+    if (read_patch_cache(cache, keys, data)) {
+        for (auto& d : data) patches.push_back(std::make_unique<SurfacePatch>(std::move(d)));
+        if (from_cache) *from_cache = true;
+        return patches;
+    }
+    std::vector<const SurfacePatch::Data*> ptrs;
+    for (const auto& f : files) {
+        patches.push_back(std::make_unique<SurfacePatch>(f));
+        ptrs.push_back(&patches.back()->data());
+    }
+    write_patch_cache(cache, keys, ptrs);
+    return patches;
+}
+}  // namespace detail
+
+// Directory for the binary patch cache of every driver built afterwards ("" turns it off).
+inline void set_patch_cache_directory(const std::string& dir) { detail::patch_cache_dir() = dir; }
+
 class RayTracingDriver {
 public:
     struct RayPath {
@@ -486,10 +654,8 @@ private:
         for (const auto& e : fs::directory_iterator(patch_directory))
             if (e.is_regular_file() && e.path().extension() == ".txt") files.push_back(e.path());
         std::sort(files.begin(), files.end());
-        for (const auto& p : files) {
-            patches_.push_back(std::make_unique<SurfacePatch>(p.string()));
-            paths_loaded_.push_back(p.string());
-        }
+        for (const auto& p : files) paths_loaded_.push_back(p.string());
+        patches_ = detail::load_patch_list(paths_loaded_);
         if (patches_.empty())
             throw std::runtime_error("RayTracingDriver: no *.txt patch files found in: " + patch_directory);
     }
@@ -499,10 +665,8 @@ private:
         if (last < first)
             throw std::invalid_argument("RayTracingDriver: empty patch index range [" + std::to_string(first) + ", " +
                                         std::to_string(last) + "]");
-        for (int i = first; i <= last; ++i) {
-            paths_loaded_.push_back(directory + prefix + std::to_string(i) + ".txt");
-            patches_.push_back(std::make_unique<SurfacePatch>(paths_loaded_.back()));
-        }
+        for (int i = first; i <= last; ++i) paths_loaded_.push_back(directory + prefix + std::to_string(i) + ".txt");
+        patches_ = detail::load_patch_list(paths_loaded_);
     }
 
     void build_scene() {

@@ -5,10 +5,14 @@
 //   drivers_check trace <dir> <rays.bin> <n> <max_bounces> <newton_mode> <out>
 //   drivers_check plane <dir> c0x c0y c0z c1x c1y c1z dx dy dz nx ny max_bounces count_direct <out>
 //   drivers_check bmap  <dir> <prefix> <first> <last> <n> <theta_n> <phi_n> <cone_deg> <max_bounces> <capture> <out>
+//   drivers_check cache <dir> <cache_dir>   (host only: the binary patch cache round trip and its staleness check)
 //
 // Every run writes <out>.zone: int64 zoneID of each patch in the driver's
 // patch order (lexicographic file order).
+#include <algorithm>
 #include <cstdio>
+#include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <string>
@@ -59,6 +63,39 @@ int main(int argc, char** argv) {
     try {
         if (argc < 2) throw std::invalid_argument("usage: drivers_check trace|plane|bmap ...");
         const std::string cmd = argv[1];
+        if (cmd == "cache" && argc == 4) {
+            namespace fs = std::filesystem;
+            set_patch_cache_directory(argv[3]);
+            std::vector<std::string> files;
+            for (const auto& e : fs::directory_iterator(argv[2]))
+                if (e.is_regular_file() && e.path().extension() == ".txt") files.push_back(e.path().string());
+            std::sort(files.begin(), files.end());
+            bool c1 = true, c2 = false, c3 = true;
+            const auto a = detail::load_patch_list(files, &c1);   // parsed, cache written
+            const auto b = detail::load_patch_list(files, &c2);   // from the cache
+            auto same = [](const SurfacePatch::Data& x, const SurfacePatch::Data& y) {
+                auto eq = [](const auto& u, const auto& v) {
+                    return u.size() == v.size() &&
+                           std::memcmp(u.data(), v.data(), u.size() * sizeof(u[0])) == 0;
+                };
+                return x.imax == y.imax && x.jmax == y.jmax && x.kmax == y.kmax && x.zoneID == y.zoneID &&
+                       eq(x.uNodes, y.uNodes) && eq(x.vNodes, y.vNodes) && eq(x.uWeights, y.uWeights) &&
+                       eq(x.vWeights, y.vWeights) && eq(x.x, y.x) && eq(x.y, y.y) && eq(x.z, y.z) &&
+                       eq(x.mask, y.mask) && eq(x.dxdu, y.dxdu) && eq(x.dydu, y.dydu) && eq(x.dzdu, y.dzdu) &&
+                       eq(x.dxdv, y.dxdv) && eq(x.dydv, y.dydv) && eq(x.dzdv, y.dzdv) && eq(x.dS, y.dS) &&
+                       eq(x.nuX, y.nuX) && eq(x.nuY, y.nuY) && eq(x.nuZ, y.nuZ) && x.edge_flags == y.edge_flags;
+            };
+            bool equal = a.size() == b.size() && !a.empty();
+            for (size_t k = 0; equal && k < a.size(); ++k) {
+                equal = same(a[k]->data(), b[k]->data()) && a[k]->centroid().x == b[k]->centroid().x &&
+                        a[k]->centroid().y == b[k]->centroid().y && a[k]->centroid().z == b[k]->centroid().z;
+            }
+            { std::ofstream(files[0], std::ios::app) << "\n"; }   // the first file changes: the cache is stale
+            const auto c = detail::load_patch_list(files, &c3);
+            std::cout << "cache patches=" << a.size() << " first_from_cache=" << c1 << " second_from_cache=" << c2
+                      << " after_change_from_cache=" << c3 << " equal=" << equal << std::endl;
+            return (!c1 && c2 && !c3 && equal && c.size() == a.size()) ? 0 : 1;
+        }
         if (cmd == "trace" && argc == 8) {
             RayTracingDriver drv(argv[2]);
             drv.newton_mode = std::stoi(argv[6]);
