@@ -4,7 +4,11 @@ Workload (default): the Sinai-like scene -- 4 spheres + 1 ellipsoid, 2,048
 order-4 (5x5 tensor) patches each, 10,240 patches -- and 2^20 rays from the
 z=-8 plane in a 10 degree cone, max 100 bounces, FP64 Newton in the
 `converged` mode, triangle (stage-1) seeds.  One step = one trace of the whole
-ray batch, inputs and outputs resident in HBM.
+ray batch, inputs and outputs resident in HBM.  By default two steps are kept
+in flight on two streams (bbrt_trace_device_async), each with its own outputs:
+the next step's rays start on the CUs the previous step's last rays leave.  A
+pass with one synchronous trace at a time is timed too (`one_in_flight`) and
+gives the roofline's kernel launch times.
 
 Prints one JSON line with the throughput, the FP64 roofline of the bounce
 kernel (algorithmic Newton flops / summed kernel time, HIP events on the
@@ -21,6 +25,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# Streams that share a hardware queue run their kernels in order.  With the
+# HIP default of 4 queues per process, the first two streams created after a
+# scene handle's own streams share one, which would serialise the in-flight
+# traces; 8 queues give each of them its own.  Set before the HIP runtime starts.
+os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 METRIC = "converged bounces/s at ≤1e-10 hit error; mean Newton iters, learned vs tri seed"
 FP64_PEAK_TFLOPS = 78.6
@@ -38,6 +47,9 @@ def main():
     ap.add_argument("--mode", choices=["ref", "converged"], default="converged")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle time budget (0 = skip)")
+    ap.add_argument("--in-flight", type=int, default=2,
+                    help="Sinai headline: traces kept in flight on that many streams (bbrt_trace_device_async); "
+                         "1 = one synchronous trace at a time")
     ap.add_argument("--no-cull", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-eu", type=int, default=0)
@@ -58,6 +70,8 @@ def main():
     ap.add_argument("--no-full-form", action="store_true", help="skip the full batched form leg (512 scenes, 16M rays)")
     ap.add_argument("--full-scenes", type=int, default=512)
     ap.add_argument("--full-rays", type=int, default=16_000_000)
+    ap.add_argument("--full-in-flight", type=int, default=2,
+                    help="full form: traces kept in flight (bbrt_trace_device_async); 1 = graph replays only")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -136,6 +150,22 @@ def main():
     kernel_ms = kms_tot[0]
     flops = float(ctr_tot[L.CTR["newton_flops"]]) + 27.0 * bounces
     achieved_tflops = flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+    # The pass above runs one synchronous trace at a time: its HIP-event launch times give the roofline.
+    # The headline then keeps in_flight traces in flight on as many streams (bbrt_trace_device_async), so a
+    # step's first rays start on the CUs the previous step's last rays have left; every step is the whole
+    # trace of all 2^20 rays, with its own outputs and device counters.
+    one_in_flight = {"value": value, "ms_per_step": ms_per_step, "steps": args.steps,
+                     "note": "one synchronous trace at a time; the roofline's launch times come from this pass"}
+    in_flight = 1 if args.wavefront else max(1, args.in_flight)
+    in_flight_check = None
+    if in_flight > 1:
+        dt, cd, in_flight_check = run_in_flight(
+            gs, to, td, n, mb, outs, args.steps, args.warmup, in_flight, ctr_tot // max(1, args.steps),
+            dict(mode=mode, record_k=K, flags=flags & ~L.FLAG_TIME_KERNELS, block=args.block,
+                 waves_per_eu=args.waves_per_eu, newton_batch=args.newton_batch))
+        bounces = int(cd[:, L.CTR["bounces"]].sum())
+        value = bounces / dt
+        ms_per_step = dt * 1e3 / args.steps
     newton_calls = int(ctr_tot[L.CTR["newton_calls"]])
     newton_iters = int(ctr_tot[L.CTR["newton_iters"]])
     per_step = {k: int(ctr_tot[v] // args.steps) for k, v in L.CTR.items()}
@@ -154,6 +184,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "sinai-5-obstacles-p4-10240-patches", "rays": n, "patches": sc.n_patches,
+                   "traces_in_flight": in_flight,
                    "max_bounces": mb, "record_k": K, "newton_mode": args.mode, "seed_mode": "triangle",
                    "culling": not args.no_cull, "scene_seed": args.seed,
                    "schedule": "wavefront" if args.wavefront else "persistent", "block": args.block or 256,
@@ -164,6 +195,8 @@ def main():
                      "kernel": "k_bounce" if args.wavefront else "k_persist", "launches_per_step": launches / args.steps,
                      "avg_launch_ms": kernel_ms / launches if launches else None,
                      "flops_per_launch": flops / launches if launches else None},
+        "one_in_flight": one_in_flight,
+        "in_flight_check": in_flight_check,
         "mean_newton_iters": {"triangle": newton_iters / newton_calls if newton_calls else None, "learned": None},
         "bounces_per_step": bounces // args.steps,
         "rays_by_status": status_counts(outs["status"]),
@@ -176,7 +209,8 @@ def main():
         if mode != L.NEWTON_CONVERGED or args.wavefront:
             res["learned"] = {"skipped": "the learned seed runs with converged Newton in the persistent schedule"}
         else:
-            res["learned"] = learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value, res)
+            res["learned"] = learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags,
+                                         one_in_flight["value"], res)
             res["mean_newton_iters"]["learned"] = res["learned"]["mean_newton_iters"]
 
     traffic = committed_traffic(args, n, mb, K)
@@ -268,6 +302,45 @@ def batch_leg(args):
 ERROR_KINDS = ("none", "newton_singular", "newton_noconv", "newton_domain", "backface", "zero_dir", "stack")
 
 
+def run_in_flight(g, to, td, n, mb, outs, steps, warmup, in_flight, ref_counters, trace_kw):
+    """Times `steps` traces kept `in_flight` deep on as many streams (bbrt_trace_device_async).  Each stream
+    writes its own outputs (the first set is `outs`), each trace its own device counters.  Returns (seconds,
+    counters [steps][NCOUNTERS], check); exits if a trace's counters differ from ref_counters (a synchronous
+    trace's) or an output set differs from `outs`."""
+    import torch
+    from billardballraytracing3d_amd import _lib as L
+
+    streams = [torch.cuda.Stream(to.device) for _ in range(in_flight)]
+    extra = [{k: torch.empty_like(v) for k, v in outs.items()} for _ in range(in_flight - 1)]
+    sets = [{k: v.data_ptr() for k, v in outs.items()}] + [{k: v.data_ptr() for k, v in e.items()} for e in extra]
+    ctr = torch.zeros((max(1, steps), L.NCOUNTERS), dtype=torch.int64, device=to.device)
+
+    def astep(i):
+        g.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, sets[i % in_flight],
+                             counters_ptr=ctr[i % ctr.shape[0]].data_ptr(), stream=streams[i % in_flight].cuda_stream,
+                             **trace_kw)
+
+    for i in range(max(warmup, in_flight)):
+        astep(i)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for i in range(steps):
+        astep(i)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - ts
+    cd = ctr.cpu().numpy()[:steps]
+    # launches and the network's wave-level evaluation counts depend on scheduling, not on the rays
+    sched = ("launches", "mlp_evals", "mlp_lanes")
+    differ = [name for name, q in L.CTR.items() if name not in sched and bool((cd[:, q] != ref_counters[q]).any())]
+    check = {"traces_in_flight": in_flight, "counters_equal_synchronous": not differ,
+             "outputs_equal_across_streams": all(torch.equal(e[k], outs[k]) for e in extra for k in outs)}
+    if differ:
+        check["counters_differing"] = differ
+    if not (check["counters_equal_synchronous"] and check["outputs_equal_across_streams"]):
+        raise SystemExit(f"in-flight traces differ from the synchronous trace: {check}")
+    return dt, cd, check
+
+
 def error_counts(err_kind):
     """Rays of the last step per error kind (device int8 array)."""
     import torch
@@ -344,7 +417,19 @@ def full_form_leg(args):
     torch.cuda.synchronize()
     te = time.perf_counter()
     bounces = int(ctr[L.CTR["bounces"]])
-    _progress(f"full form: {steps} steps, {(te - ts) * 1e3 / steps:.1f} ms/step, {bounces / (te - ts):.4g} bounces/s")
+    _progress(f"full form: {steps} steps, {(te - ts) * 1e3 / steps:.1f} ms/step, {bounces / (te - ts):.4g} bounces/s"
+              " (graph replay, one in flight)")
+    one_in_flight = {"value": bounces / (te - ts), "ms_per_step": (te - ts) * 1e3 / steps, "steps": steps,
+                     "note": "one hipGraph-replayed trace at a time"}
+    in_flight_check = None
+    if args.full_in_flight > 1:
+        dt, cd, in_flight_check = run_in_flight(
+            gb, to, td, n, mb, outs, steps, max(1, args.warmup), args.full_in_flight, ctr // steps,
+            dict(record_k=K, ray_begin=rb, seed=L.SEED_LEARNED, **best))
+        bounces = int(cd[:, L.CTR["bounces"]].sum())
+        ts, te = 0.0, dt
+        _progress(f"full form: {steps} steps, {dt * 1e3 / steps:.1f} ms/step, {bounces / dt:.4g} bounces/s"
+                  f" ({args.full_in_flight} in flight)")
     # fallback rate per scene
     ss = outs["seed_stats"].cpu().numpy().astype(np.int64)
     per = []
@@ -379,7 +464,9 @@ def full_form_leg(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"batched-full-form-{len(scenes)}-scenes-{n}-rays", "scenes": len(scenes),
                    "patches": gb.n_patches, "rays": n, "max_bounces": mb, "record_k": K, "newton_mode": "converged",
-                   "seed_mode": "learned", "scene": "compact, device build", "autotuned": best},
+                   "seed_mode": "learned", "scene": "compact, device build", "autotuned": best,
+                   "traces_in_flight": max(1, args.full_in_flight)},
+        "one_in_flight": one_in_flight, "in_flight_check": in_flight_check,
         "mean_newton_iters": {"learned": int(ctr[L.CTR["newton_iters"]]) / hits if hits else None},
         "seed_learned_per_step": int(ctr[L.CTR["seed_learned"]]) // steps,
         "seed_fallback_per_step": int(ctr[L.CTR["seed_fallback"]]) // steps,

@@ -617,6 +617,13 @@ __global__ void __launch_bounds__(256) k_patch_pyramid(DevScene S, int64_t p0, d
     }
 }
 
+// The counters of an asynchronous trace, into the caller's device array
+// (zeros when ctr is NULL: an empty trace).
+__global__ void k_counters_out(const unsigned long long* __restrict__ ctr, int64_t* __restrict__ out, int64_t launches) {
+    const int q = (int)threadIdx.x;
+    if (q < BBRT_NCOUNTERS) out[q] = q == bbrt::kCtrLaunches ? launches : (ctr ? (int64_t)ctr[q] : 0);
+}
+
 // Per-ray sub-scene ids from the sub-scene ray ranges (binary search).
 __global__ void k_ray_scene(const int64_t* __restrict__ begin, int n_scenes, int64_t n, int32_t* __restrict__ out) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -629,10 +636,15 @@ __global__ void k_ray_scene(const int64_t* __restrict__ begin, int n_scenes, int
     out[g] = lo;
 }
 
+// async: enqueue on st and return without waiting (bbrt_trace_device_async):
+// the stream's own scratch, counters (if d_counters) written on the device.
 void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, const bbrt_params* p,
                 const bbrt_outputs* out, int64_t* counters, hipStream_t st, double* kernel_ms, bool pad_records,
-                const int32_t* d_ray_scene) {
+                const int32_t* d_ray_scene, bool async = false, int64_t* d_counters = nullptr) {
     if (!p) throw std::runtime_error("bbrt_trace: params is NULL");
+    if (async && (p->flags & (BBRT_FLAG_WAVEFRONT | BBRT_FLAG_GRAPH | BBRT_FLAG_TIME_KERNELS)))
+        throw std::runtime_error("bbrt_trace_device_async: BBRT_FLAG_WAVEFRONT, BBRT_FLAG_GRAPH and "
+                                 "BBRT_FLAG_TIME_KERNELS need the synchronous entry points");
     if (s->n_scenes > 1 && !d_ray_scene)
         throw std::runtime_error("bbrt_trace: this scene holds several sub-scenes; use bbrt_trace_batch");
     DevScene dev = s->dev;
@@ -658,6 +670,11 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     check(hipSetDevice(s->device), "hipSetDevice");
     int64_t local_ctr[BBRT_NCOUNTERS] = { 0 };
     if (n == 0) {
+        if (async) {
+            if (d_counters) hipLaunchKernelGGL(k_counters_out, dim3(1), dim3(64), 0, st, nullptr, d_counters, (int64_t)0);
+            check(hipGetLastError(), "k_counters_out");
+            return;
+        }
         if (counters) std::memcpy(counters, local_ctr, sizeof(local_ctr));
         if (kernel_ms) { kernel_ms[0] = 0; kernel_ms[1] = 0; kernel_ms[2] = 0; }
         return;
@@ -670,7 +687,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     const size_t off_cnt = align256(off_lb + (size_t)n * 4);
     const size_t off_ctr = align256(off_cnt + 256);
     const size_t need = align256(off_ctr + BBRT_NCOUNTERS * 8);
-    char* base = static_cast<char*>(scratch_for(s, need));
+    char* base = static_cast<char*>(async ? async_scratch_for(s, st, need) : scratch_for(s, need));
     RayState R{ reinterpret_cast<double*>(base + off_o), reinterpret_cast<double*>(base + off_d) };
     int32_t* live[2] = { reinterpret_cast<int32_t*>(base + off_la), reinterpret_cast<int32_t*>(base + off_lb) };
     int32_t* cnt = reinterpret_cast<int32_t*>(base + off_cnt);   // [0], [1]: list sizes, [2]: queue head
@@ -743,6 +760,13 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             if (events) check(hipEventRecord(s->kev[1], q), "hipEventRecord");
         }
     };
+    if (async) {
+        enqueue(st, false);
+        if (d_counters)
+            hipLaunchKernelGGL(k_counters_out, dim3(1), dim3(64), 0, st, ctr, d_counters, (int64_t)(kern ? 1 : 0));
+        check(hipGetLastError(), "bounce kernel launch");
+        return;
+    }
     int launches = 0;
     bool graph_timed = false;
     if (graph) {
@@ -1075,6 +1099,8 @@ const int32_t* ray_scene_ids(bbrt_scene s, const int64_t* begin, hipStream_t st)
     const int64_t n = begin[s->n_scenes];
     std::vector<int64_t> key(begin, begin + s->n_scenes + 1);
     if (key == s->ray_scene_key && s->d_ray_scene) return s->d_ray_scene;
+    // traces still in flight on any stream may read the current ids
+    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (n > s->ray_scene_cap) {
         if (s->d_ray_scene) check(hipFree(s->d_ray_scene), "hipFree");
         s->d_ray_scene = nullptr;
@@ -1259,6 +1285,28 @@ int bbrt_trace_device(bbrt_scene s, const double* origins, const double* dirs, i
     if (!s) { g_last_error = "bbrt_trace_device: NULL scene"; return -1; }
     try {
         trace_impl(s, origins, dirs, n, p, out, counters, static_cast<hipStream_t>(stream), kernel_ms, false, nullptr);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int bbrt_trace_device_async(bbrt_scene s, const double* origins, const double* dirs, int64_t n,
+                            const int64_t* ray_begin, const bbrt_params* p, const bbrt_outputs* out,
+                            int64_t* d_counters, void* stream) {
+    if (!s) { g_last_error = "bbrt_trace_device_async: NULL scene"; return -1; }
+    try {
+        std::lock_guard<std::mutex> lock(s->async_mu);
+        check(hipSetDevice(s->device), "hipSetDevice");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+        const int32_t* ids = nullptr;
+        if (ray_begin) {
+            check_ranges(s, ray_begin);
+            if (ray_begin[s->n_scenes] != n)
+                throw std::runtime_error("bbrt_trace_device_async: n_rays must equal ray_begin[n_scenes]");
+            ids = ray_scene_ids(s, ray_begin, st);
+        }
+        trace_impl(s, origins, dirs, n, p, out, nullptr, st, nullptr, false, ids, true, d_counters);
         return 0;
     } catch (const std::exception& e) {
         return fail(e);

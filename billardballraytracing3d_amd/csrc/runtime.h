@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -73,6 +74,16 @@ struct bbrt_scene_s {
     // per-trace scratch, grown on demand
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // bbrt_trace_device_async: one scratch buffer per stream, so traces on
+    // distinct streams can be in flight together; the mutex guards this list
+    // and the per-ray sub-scene ids
+    struct StreamScratch {
+        hipStream_t stream;
+        void* p;
+        size_t bytes;
+    };
+    std::vector<StreamScratch> async_scratch;
+    std::mutex async_mu;
     int32_t* h_count = nullptr;   // pinned
     // BBRT_FLAG_GRAPH: the captured trace and the arguments it was captured with
     hipStream_t gstream = nullptr;
@@ -105,6 +116,7 @@ struct bbrt_scene_s {
         for (void* p : allocs) (void)hipFree(p);
         if (d_ray_scene) (void)hipFree(d_ray_scene);
         if (scratch) (void)hipFree(scratch);
+        for (auto& a : async_scratch) if (a.p) (void)hipFree(a.p);
         if (h_count) (void)hipHostFree(h_count);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (ev_dep) (void)hipEventDestroy(ev_dep);
@@ -132,6 +144,27 @@ void* scratch_for(bbrt_scene s, size_t need) {
         s->scratch_bytes = need;
     }
     return s->scratch;
+}
+
+// The async scratch of `st`, grown on demand.  Growing waits for the stream
+// first: its earlier traces may still use the old buffer.
+void* async_scratch_for(bbrt_scene s, hipStream_t st, size_t need) {
+    for (auto& a : s->async_scratch) {
+        if (a.stream != st) continue;
+        if (need > a.bytes) {
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
+            check(hipFree(a.p), "hipFree(scratch)");
+            a.p = nullptr;
+            a.bytes = 0;
+            check(hipMalloc(&a.p, need), "hipMalloc(scratch)");
+            a.bytes = need;
+        }
+        return a.p;
+    }
+    void* p = nullptr;
+    check(hipMalloc(&p, need), "hipMalloc(scratch)");
+    s->async_scratch.push_back({ st, p, need });
+    return p;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }

@@ -199,6 +199,21 @@ class GpuScene:
             L.check(rc, "bbrt_trace_batch_device")
         return counters, kms
 
+    def trace_device_async(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
+                           counters_ptr: int = 0, mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0,
+                           stream: int = 0, block: int = 0, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE,
+                           newton_batch: int = 0, grid_fraction: float = 0.0, ray_begin=None):
+        """bbrt_trace_device_async: enqueue the trace on `stream` and return at once.  counters_ptr: 0 or a
+        device address of NCOUNTERS int64, written on the stream.  Traces on distinct streams may overlap."""
+        p = _params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newton_batch, grid_fraction)
+        out = _outputs(out_ptrs)
+        rb = None if ray_begin is None else np.ascontiguousarray(ray_begin, dtype=np.int64)
+        rc = L.lib().bbrt_trace_device_async(self.h, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), C.c_int64(n),
+                                             L.ptr(rb, L._i64) if rb is not None else None, C.byref(p),
+                                             C.byref(out), C.c_void_p(counters_ptr) if counters_ptr else None,
+                                             C.c_void_p(stream) if stream else None)
+        L.check(rc, "bbrt_trace_device_async")
+
     def autotune(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
                  mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0, stream: int = 0,
                  seed: int = L.SEED_TRIANGLE, ray_begin=None):

@@ -196,6 +196,27 @@ int bbrt_trace_batch_device(bbrt_scene scene, const double* origins, const doubl
                             const bbrt_params* params, const bbrt_outputs* out, int64_t* counters, void* stream,
                             double* kernel_ms);
 
+/* bbrt_trace_device / bbrt_trace_batch_device without waiting: enqueues the
+ * trace on `stream` (NULL = the scene's own stream) and returns.  Replaces the
+ * same ray loops as bbrt_trace_device (RayTracingDriver.hpp:119-150); the
+ * reference has no asynchronous form.
+ * ray_begin: NULL for a one-scene handle, else as for bbrt_trace_batch (host
+ * array, read during the call), with n = ray_begin[n_scenes].
+ * d_counters: NULL or device memory for BBRT_NCOUNTERS int64, written on the
+ * stream when the trace ends.
+ * Each stream gets its own scratch in the handle, so traces enqueued on
+ * distinct streams may run at the same time: the next batch's rays start on
+ * CUs the previous batch's last rays have left.  Calls on one stream run in
+ * order.  Inputs and outputs must stay valid, and the handle alive, until the
+ * stream has finished the trace.  Calls may come from several host threads
+ * (they serialise on the handle for the enqueue only), but not concurrently
+ * with the handle's synchronous calls.  BBRT_FLAG_WAVEFRONT, BBRT_FLAG_GRAPH
+ * and BBRT_FLAG_TIME_KERNELS are refused (they need the host to wait).
+ * Changing ray_begin between calls waits for the whole device once. */
+int bbrt_trace_device_async(bbrt_scene scene, const double* origins, const double* dirs, int64_t n,
+                            const int64_t* ray_begin, const bbrt_params* params, const bbrt_outputs* out,
+                            int64_t* d_counters, void* stream);
+
 /* Launch-shape autotuner for the persistent kernel.  Times the trace of the
  * given device-resident rays (ray_begin as for bbrt_trace_batch, or NULL and n
  * rays of a single scene; outputs in `out` are overwritten) under a coordinate

@@ -1,0 +1,131 @@
+"""bbrt_trace_device_async: traces enqueued without waiting, several in flight on distinct streams.
+
+* every trace gives, record for record and counter for counter, what the synchronous call gives;
+* traces on two streams with their own outputs overlap and stay independent;
+* a smaller trace after a larger one on the same stream reuses the stream's scratch;
+* a batch handle with changed ray ranges, and an empty trace, are handled;
+* the flags that need the host to wait are refused.
+"""
+import numpy as np
+import pytest
+import torch
+
+from billardballraytracing3d_amd import _lib as L
+from billardballraytracing3d_amd import scenes as S
+from billardballraytracing3d_amd import tracer as T
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def _outs(n, K):
+    return dict(status=torch.full((n,), 9, dtype=torch.int8, device=DEV),
+                err_kind=torch.full((n,), 9, dtype=torch.int8, device=DEV),
+                bounces=torch.full((n,), -9, dtype=torch.int32, device=DEV),
+                err_patch=torch.full((n,), -9, dtype=torch.int32, device=DEV),
+                final_od=torch.zeros((n, 6), dtype=torch.float64, device=DEV),
+                hit_patch=torch.full((n, K), -7, dtype=torch.int32, device=DEV),
+                hit_xyz=torch.full((n, K, 3), 0.5, dtype=torch.float64, device=DEV),
+                newton_iters=torch.zeros((n, K), dtype=torch.uint8, device=DEV))
+
+
+def _ptrs(outs):
+    return {k: v.data_ptr() for k, v in outs.items()}
+
+
+@pytest.fixture(scope="module")
+def sinai():
+    sc, o, d, mb = S.sinai_cone(n_rays=1 << 15)
+    return sc, o, d, mb, T.GpuScene(sc)
+
+
+def test_two_streams_match_synchronous(sinai):
+    sc, o, d, mb, gs = sinai
+    n, K = o.shape[0], 16
+    to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+    ref = _outs(n, K)
+    c_ref, _ = gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(ref), record_k=K)
+    torch.cuda.synchronize()
+    assert c_ref[L.CTR["bounces"]] > 0
+    streams = [torch.cuda.Stream(DEV), torch.cuda.Stream(DEV)]
+    sets = [_outs(n, K), _outs(n, K)]
+    ctr = torch.full((6, L.NCOUNTERS), -1, dtype=torch.int64, device=DEV)
+    torch.cuda.synchronize()
+    for i in range(6):   # six traces, alternating streams; each stream rewrites its own outputs
+        gs.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(sets[i % 2]),
+                              counters_ptr=ctr[i].data_ptr(), record_k=K, stream=streams[i % 2].cuda_stream)
+    torch.cuda.synchronize()
+    for s in sets:
+        for k in ref:
+            assert torch.equal(s[k], ref[k]), k
+    c = ctr.cpu().numpy()
+    for i in range(6):
+        np.testing.assert_array_equal(c[i], c_ref, err_msg=f"trace {i}")
+
+
+def test_smaller_trace_after_larger_on_one_stream(sinai):
+    sc, o, d, mb, gs = sinai
+    K = 8
+    st = torch.cuda.Stream(DEV)
+    for n in (o.shape[0], 3000, 1):
+        to, td = torch.from_numpy(o[:n]).to(DEV), torch.from_numpy(d[:n]).to(DEV)
+        ref = _outs(n, K)
+        c_ref, _ = gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(ref), record_k=K)
+        got = _outs(n, K)
+        ctr = torch.zeros(L.NCOUNTERS, dtype=torch.int64, device=DEV)
+        torch.cuda.synchronize()
+        gs.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(got), counters_ptr=ctr.data_ptr(),
+                              record_k=K, stream=st.cuda_stream)
+        st.synchronize()
+        for k in ref:
+            assert torch.equal(got[k], ref[k]), (n, k)
+        np.testing.assert_array_equal(ctr.cpu().numpy(), c_ref, err_msg=str(n))
+
+
+def test_empty_trace_writes_zero_counters(sinai):
+    gs = sinai[4]
+    ctr = torch.full((L.NCOUNTERS,), -1, dtype=torch.int64, device=DEV)
+    torch.cuda.synchronize()
+    gs.trace_device_async(0, 0, 0, 10, {}, counters_ptr=ctr.data_ptr())
+    torch.cuda.synchronize()
+    assert (ctr.cpu().numpy() == 0).all()
+
+
+def test_batch_ranges_change_between_calls():
+    scenes, o, d, rb = S.batch_config(seed=5, n_scenes=5, patch_lo=80, patch_hi=500, rays_lo=200, rays_hi=1500)
+    gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
+    mb, K = 20, 8
+    rb2 = np.concatenate([[0], np.cumsum(np.diff(rb) // 2 + 1)]).astype(np.int64)   # about half the rays
+    st = [torch.cuda.Stream(DEV), torch.cuda.Stream(DEV)]
+    for rep, r in enumerate((rb, rb2, rb)):
+        n = int(r[-1])
+        oo = np.concatenate([o[rb[i]:rb[i] + (r[i + 1] - r[i])] for i in range(len(scenes))])
+        dd = np.concatenate([d[rb[i]:rb[i] + (r[i + 1] - r[i])] for i in range(len(scenes))])
+        to, td = torch.from_numpy(oo).to(DEV), torch.from_numpy(dd).to(DEV)
+        ref = _outs(n, K)
+        c_ref, _ = gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(ref), record_k=K, ray_begin=r)
+        got = [_outs(n, K), _outs(n, K)]
+        ctr = torch.zeros((2, L.NCOUNTERS), dtype=torch.int64, device=DEV)
+        torch.cuda.synchronize()
+        for j in range(2):
+            gb.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(got[j]), counters_ptr=ctr[j].data_ptr(),
+                                  record_k=K, ray_begin=r, stream=st[j].cuda_stream)
+        torch.cuda.synchronize()
+        for j in range(2):
+            for k in ref:
+                assert torch.equal(got[j][k], ref[k]), (rep, j, k)
+            np.testing.assert_array_equal(ctr[j].cpu().numpy(), c_ref, err_msg=f"{rep} {j}")
+
+
+def test_refuses_flags_that_wait(sinai):
+    sc, o, d, mb, gs = sinai
+    n = 64
+    to, td = torch.from_numpy(o[:n]).to(DEV), torch.from_numpy(d[:n]).to(DEV)
+    outs = _outs(n, 4)
+    for f in (L.FLAG_WAVEFRONT, L.FLAG_GRAPH, L.FLAG_TIME_KERNELS):
+        with pytest.raises(L.BBRTError, match="synchronous entry points"):
+            gs.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(outs), record_k=4, flags=f)
+    with pytest.raises(L.BBRTError, match="ray_begin"):
+        gs.trace_device_async(to.data_ptr(), td.data_ptr(), n + 1, mb, _ptrs(outs), record_k=4,
+                              ray_begin=np.array([0, n]))
