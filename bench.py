@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-in-flight", type=int, default=2,
+                    help="batched leg: traces kept in flight (bbrt_trace_device_async); 1 = graph replays only")
     ap.add_argument("--batch-no-tune", action="store_true", help="batched leg: library default launch shape")
     ap.add_argument("--batch-only", action="store_true", help="run the batched-scenes leg alone")
     ap.add_argument("--workload", choices=["sinai", "full-form"], default="sinai",
@@ -286,9 +288,19 @@ def batch_leg(args):
     bounces = int(ctr[L.CTR["bounces"]])
     flops = float(ctr[L.CTR["newton_flops"]]) + 27.0 * bounces
     graph_ms = kms[0] / args.batch_steps
+    one_in_flight = {"value": bounces / (te - ts), "ms_per_step": (te - ts) * 1e3 / args.batch_steps,
+                     "note": "one hipGraph-replayed trace at a time"}
+    in_flight_check = None
+    if args.batch_in_flight > 1:
+        dt, cd, in_flight_check = run_in_flight(
+            gb, to, td, n, mb, outs, args.batch_steps, max(1, args.warmup), args.batch_in_flight,
+            ctr // args.batch_steps, dict(record_k=K, ray_begin=rb, **best))
+        ts, te = 0.0, dt
     return {
         "workload": f"batched-bounce-map-{args.batch_scenes}-scenes", "value": bounces / (te - ts),
         "unit": "bounces/s", "ms_per_step": (te - ts) * 1e3 / args.batch_steps, "steps": args.batch_steps,
+        "traces_in_flight": max(1, args.batch_in_flight), "one_in_flight": one_in_flight,
+        "in_flight_check": in_flight_check,
         "scenes": args.batch_scenes, "patches": n_patches, "rays": n, "max_bounces": mb, "record_k": K,
         "newton_mode": "converged", "scene": "compact (coarse cells evaluated on the fly), device build",
         "bounces_per_step": bounces // args.batch_steps, "graph_ms_per_step": graph_ms,
