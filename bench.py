@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--in-flight", type=int, default=2,
                     help="Sinai headline: traces kept in flight on that many streams (bbrt_trace_device_async); "
                          "1 = one synchronous trace at a time")
+    ap.add_argument("--in-flight-order", choices=["grouped", "input"], default="grouped",
+                    help="queue order of the in-flight passes (Sinai and batched legs): BBRT_FLAG_GROUP_RAYS or input")
     ap.add_argument("--no-cull", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--waves-per-eu", type=int, default=0)
@@ -163,7 +165,7 @@ def main():
     if in_flight > 1:
         dt, cd, in_flight_check = run_in_flight(
             gs, to, td, n, mb, outs, args.steps, args.warmup, in_flight, ctr_tot // max(1, args.steps),
-            dict(mode=mode, record_k=K, flags=flags & ~L.FLAG_TIME_KERNELS, block=args.block,
+            dict(mode=mode, record_k=K, flags=(flags & ~L.FLAG_TIME_KERNELS) | group_flag(args), block=args.block,
                  waves_per_eu=args.waves_per_eu, newton_batch=args.newton_batch))
         bounces = int(cd[:, L.CTR["bounces"]].sum())
         value = bounces / dt
@@ -187,6 +189,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "sinai-5-obstacles-p4-10240-patches", "rays": n, "patches": sc.n_patches,
                    "traces_in_flight": in_flight,
+                   "queue_order": (args.in_flight_order if in_flight > 1 else "input"),
                    "max_bounces": mb, "record_k": K, "newton_mode": args.mode, "seed_mode": "triangle",
                    "culling": not args.no_cull, "scene_seed": args.seed,
                    "schedule": "wavefront" if args.wavefront else "persistent", "block": args.block or 256,
@@ -294,7 +297,7 @@ def batch_leg(args):
     if args.batch_in_flight > 1:
         dt, cd, in_flight_check = run_in_flight(
             gb, to, td, n, mb, outs, args.batch_steps, max(1, args.warmup), args.batch_in_flight,
-            ctr // args.batch_steps, dict(record_k=K, ray_begin=rb, **best))
+            ctr // args.batch_steps, dict(record_k=K, ray_begin=rb, flags=group_flag(args), **best))
         ts, te = 0.0, dt
     return {
         "workload": f"batched-bounce-map-{args.batch_scenes}-scenes", "value": bounces / (te - ts),
@@ -312,6 +315,13 @@ def batch_leg(args):
 
 
 ERROR_KINDS = ("none", "newton_singular", "newton_noconv", "newton_domain", "backface", "zero_dir", "stack")
+
+
+def group_flag(args):
+    """BBRT_FLAG_GROUP_RAYS for the in-flight passes unless --in-flight-order input: grouped rays make each
+    wave's rays coherent, which pays once traces overlap (measured +3.6 % on the Sinai workload)."""
+    from billardballraytracing3d_amd import _lib as L
+    return L.FLAG_GROUP_RAYS if args.in_flight_order == "grouped" else 0
 
 
 def run_in_flight(g, to, td, n, mb, outs, steps, warmup, in_flight, ref_counters, trace_kw):

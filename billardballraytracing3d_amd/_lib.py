@@ -20,6 +20,7 @@ ESCAPED, TRAPPED, ERROR = 0, 1, 2
 ERR_NONE, ERR_SINGULAR, ERR_NOCONV, ERR_DOMAIN, ERR_BACKFACE, ERR_ZERO_DIR, ERR_STACK = range(7)
 FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS, FLAG_WAVEFRONT, FLAG_GRAPH = 1, 2, 4, 8, 16
 FLAG_STAGE1_WAVE, FLAG_STAGE1_LANE = 32, 64
+FLAG_GROUP_RAYS = 128
 SCENE_COMPACT, SCENE_DEVICE_BUILD = 1, 2
 NCOUNTERS = 16
 CTR = dict(rays=0, bounces=1, bvh_node_tests=2, candidates=3, cull_box_tests=4, mt_tests=5, newton_calls=6,

@@ -42,16 +42,41 @@ thread_local std::string bbrt::g_last_error;
 
 namespace {
 
+// BBRT_FLAG_GROUP_RAYS: a live ray's group bin is the cell of its scene's root
+// box (2^bits per axis, Morton order) that holds the ray point closest to the
+// box centre, sub-scene major.  The persistent kernel's queue takes the bins in
+// order, so the rays a wave takes together pass the same part of the scene.
+constexpr int kGroupBitsOne = 5;    // one scene: 32^3 bins
+constexpr int kGroupBitsBatch = 2;  // per sub-scene: 4^3 bins
+__device__ __forceinline__ int group_bin(const bbrt::BvhNode& rt, const V3& o, const V3& d, int bits, int scn) {
+    const double c[3] = { 0.5 * (rt.lo[0] + rt.hi[0]), 0.5 * (rt.lo[1] + rt.hi[1]), 0.5 * (rt.lo[2] + rt.hi[2]) };
+    const double s = (c[0] - o.x) * d.x + (c[1] - o.y) * d.y + (c[2] - o.z) * d.z;
+    const double t = s > 0.0 ? s : 0.0;
+    const double p[3] = { o.x + d.x * t, o.y + d.y * t, o.z + d.z * t };
+    const int L = 1 << bits;
+    int code = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double f = (p[a] - rt.lo[a]) / (rt.hi[a] - rt.lo[a]) * L;
+        const int q = (f >= 0.0 && f < (double)L) ? (int)f : (f >= (double)L ? L - 1 : 0);   // NaN: 0
+        for (int b = 0; b < bits; ++b) code |= ((q >> b) & 1) << (3 * b + a);
+    }
+    return (scn << (3 * bits)) | code;
+}
+
 // RayTracingDriver::trace_ray, lines 172-184: zero-direction check and
 // normalize_dir (:398-401); then the root box test of the first BVH query
 // (BVH.hpp:91-97 with best_t = inf), so rays that miss the whole scene leave
-// here as Escaped after one node test.  The rest enter the live list.
+// here as Escaped after one node test.  The rest enter the live list; with
+// group_bits > 0 each also gets its group bin (bin_out, at its list position)
+// and is counted in hist.
 __global__ void __launch_bounds__(1024) k_init(const double* __restrict__ origins, const double* __restrict__ dirs, int64_t n,
                        DevScene S, RayState R, Outs O, int max_bounces, int32_t* live_out,
-                       int32_t* n_out, unsigned long long* ctr) {
+                       int32_t* n_out, unsigned long long* ctr, int group_bits, int32_t* bin_out, int32_t* hist) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
     int32_t r = (int32_t)g;
+    int bin = 0;
     unsigned long long root_tests = 0;
     if (g < n) {
         const V3 o = { origins[3 * g], origins[3 * g + 1], origins[3 * g + 2] };
@@ -85,6 +110,11 @@ __global__ void __launch_bounds__(1024) k_init(const double* __restrict__ origin
                 R.o[3 * g] = o.x; R.o[3 * g + 1] = o.y; R.o[3 * g + 2] = o.z;
                 R.d[3 * g] = d.x; R.d[3 * g + 1] = d.y; R.d[3 * g + 2] = d.z;
                 alive = true;
+                if (group_bits > 0) {
+                    const int scn = ray_scene_of(S, g);
+                    bin = group_bin(S.bvh[S.scenes[scn].root], o, d, group_bits, scn);
+                    atomicAdd(hist + bin, 1);
+                }
             }
         }
     }
@@ -112,8 +142,39 @@ __global__ void __launch_bounds__(1024) k_init(const double* __restrict__ origin
     if (alive) {
         int off = block_base;
         for (int w = 0; w < wid; ++w) off += w_live[w];
-        live_out[off + (int)__popcll(mask & ((1ull << lane) - 1ull))] = r;
+        off += (int)__popcll(mask & ((1ull << lane) - 1ull));
+        live_out[off] = r;
+        if (group_bits > 0) bin_out[off] = bin;
     }
+}
+
+// Exclusive scan of the group bins' counts: start[b] = rays in bins before b.
+__global__ void __launch_bounds__(1024) k_scan_bins(const int32_t* __restrict__ hist, int32_t* __restrict__ start,
+                                                    int nbins) {
+    __shared__ int part[1024];
+    const int t = (int)threadIdx.x;
+    const int per = (nbins + 1023) / 1024;
+    const int b0 = min(nbins, t * per), b1 = min(nbins, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += hist[b];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t ? part[t - 1] : 0;
+    for (int b = b0; b < b1; ++b) { start[b] = run; run += hist[b]; }
+}
+
+// The live list grouped by bin: each ray to the next free place of its bin.
+__global__ void k_group_rays(const int32_t* __restrict__ live, const int32_t* __restrict__ bin,
+                             const int32_t* __restrict__ n_live, int32_t* cursor, int32_t* __restrict__ grouped) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= *n_live) return;
+    grouped[atomicAdd(cursor + bin[i], 1)] = live[i];
 }
 
 // Persistent bounce loop as a per-lane state machine under a wave-uniform
@@ -684,13 +745,27 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     const size_t off_d = align256(off_o + (size_t)n * 24);
     const size_t off_la = align256(off_d + (size_t)n * 24);
     const size_t off_lb = align256(off_la + (size_t)n * 4);
-    const size_t off_cnt = align256(off_lb + (size_t)n * 4);
+    // BBRT_FLAG_GROUP_RAYS (persistent schedule): bins per list position, bin counts, bin cursors
+    int group_bits = 0;
+    int64_t nbins = 0;
+    if ((p->flags & BBRT_FLAG_GROUP_RAYS) && !(p->flags & BBRT_FLAG_WAVEFRONT) && p->max_bounces > 0) {
+        group_bits = s->n_scenes > 1 ? kGroupBitsBatch : kGroupBitsOne;
+        nbins = (int64_t)s->n_scenes << (3 * group_bits);
+        if (nbins > ((int64_t)1 << 22)) { group_bits = 0; nbins = 0; }   // too many sub-scenes: input order
+    }
+    const size_t off_bin = align256(off_lb + (size_t)n * 4);
+    const size_t off_hist = align256(off_bin + (group_bits ? (size_t)n * 4 : 0));
+    const size_t off_cur = align256(off_hist + (size_t)nbins * 4);
+    const size_t off_cnt = align256(off_cur + (size_t)nbins * 4);
     const size_t off_ctr = align256(off_cnt + 256);
     const size_t need = align256(off_ctr + BBRT_NCOUNTERS * 8);
     char* base = static_cast<char*>(async ? async_scratch_for(s, st, need) : scratch_for(s, need));
     RayState R{ reinterpret_cast<double*>(base + off_o), reinterpret_cast<double*>(base + off_d) };
     int32_t* live[2] = { reinterpret_cast<int32_t*>(base + off_la), reinterpret_cast<int32_t*>(base + off_lb) };
     int32_t* cnt = reinterpret_cast<int32_t*>(base + off_cnt);   // [0], [1]: list sizes, [2]: queue head
+    int32_t* bins = reinterpret_cast<int32_t*>(base + off_bin);
+    int32_t* hist = reinterpret_cast<int32_t*>(base + off_hist);
+    int32_t* cursor = reinterpret_cast<int32_t*>(base + off_cur);
     unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base + off_ctr);
     Outs O{ out ? out->status : nullptr, out ? out->err_kind : nullptr, out ? out->bounces : nullptr,
             out ? out->err_patch : nullptr, out ? out->hit_patch : nullptr, out ? out->hit_xyz : nullptr,
@@ -751,11 +826,18 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             const unsigned pg = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
             hipLaunchKernelGGL(k_pad_records, dim3(pg), dim3(256), 0, q, O, total);
         }
+        if (group_bits)
+            hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, q, hist, (int)nbins);
         hipLaunchKernelGGL(k_init, dim3((unsigned)grid0), dim3(1024), 0, q, d_o, d_d, n, dev, R, O,
-                           p->max_bounces, live[0], cnt, ctr);
+                           p->max_bounces, live[0], cnt, ctr, group_bits, bins, hist);
+        if (group_bits) {
+            hipLaunchKernelGGL(k_scan_bins, dim3(1), dim3(1024), 0, q, hist, cursor, (int)nbins);
+            hipLaunchKernelGGL(k_group_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, q, live[0], bins, cnt,
+                               cursor, live[1]);
+        }
         if (kern) {
             if (events) check(hipEventRecord(s->kev[0], q), "hipEventRecord");
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, q, dev, R, live[0], cnt, cnt + 2, O,
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, q, dev, R, group_bits ? live[1] : live[0], cnt, cnt + 2, O,
                                p->max_bounces, p->record_k, p->newton_mode, p->flags, ctr, MD, nbatch);
             if (events) check(hipEventRecord(s->kev[1], q), "hipEventRecord");
         }

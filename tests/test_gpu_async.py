@@ -129,3 +129,44 @@ def test_refuses_flags_that_wait(sinai):
     with pytest.raises(L.BBRTError, match="ray_begin"):
         gs.trace_device_async(to.data_ptr(), td.data_ptr(), n + 1, mb, _ptrs(outs), record_k=4,
                               ray_begin=np.array([0, n]))
+
+
+def test_grouped_queue_changes_nothing(sinai):
+    """BBRT_FLAG_GROUP_RAYS only reorders the persistent kernel's queue: records and counters stay the same,
+    synchronous and in flight, on a one-scene handle and on a compact batch handle."""
+    sc, o, d, mb, gs = sinai
+    n, K = o.shape[0], 12
+    to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+    ref, got = _outs(n, K), _outs(n, K)
+    c_ref, _ = gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(ref), record_k=K)
+    c_got, _ = gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(got), record_k=K, flags=L.FLAG_GROUP_RAYS)
+    torch.cuda.synchronize()
+    for k in ref:
+        assert torch.equal(got[k], ref[k]), k
+    np.testing.assert_array_equal(c_got, c_ref)
+    st = [torch.cuda.Stream(DEV), torch.cuda.Stream(DEV)]
+    sets = [_outs(n, K), _outs(n, K)]
+    ctr = torch.zeros((4, L.NCOUNTERS), dtype=torch.int64, device=DEV)
+    torch.cuda.synchronize()
+    for i in range(4):
+        gs.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(sets[i % 2]), counters_ptr=ctr[i].data_ptr(),
+                              record_k=K, flags=L.FLAG_GROUP_RAYS, stream=st[i % 2].cuda_stream)
+    torch.cuda.synchronize()
+    for s in sets:
+        for k in ref:
+            assert torch.equal(s[k], ref[k]), k
+    for i in range(4):
+        np.testing.assert_array_equal(ctr[i].cpu().numpy(), c_ref)
+
+    scenes, ob, db, rb = S.batch_config(seed=6, n_scenes=4, patch_lo=80, patch_hi=400, rays_lo=300, rays_hi=1500)
+    gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
+    nb = int(rb[-1])
+    tob, tdb = torch.from_numpy(ob).to(DEV), torch.from_numpy(db).to(DEV)
+    ref, got = _outs(nb, K), _outs(nb, K)
+    c_ref, _ = gb.trace_device(tob.data_ptr(), tdb.data_ptr(), nb, 20, _ptrs(ref), record_k=K, ray_begin=rb)
+    c_got, _ = gb.trace_device(tob.data_ptr(), tdb.data_ptr(), nb, 20, _ptrs(got), record_k=K, ray_begin=rb,
+                               flags=L.FLAG_GROUP_RAYS | L.FLAG_GRAPH)
+    torch.cuda.synchronize()
+    for k in ref:
+        assert torch.equal(got[k], ref[k]), k
+    np.testing.assert_array_equal(c_got[:10], c_ref[:10])
