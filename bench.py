@@ -447,7 +447,7 @@ def full_form_leg(args):
     if args.full_in_flight > 1:
         dt, cd, in_flight_check = run_in_flight(
             gb, to, td, n, mb, outs, steps, max(1, args.warmup), args.full_in_flight, ctr // steps,
-            dict(record_k=K, ray_begin=rb, seed=L.SEED_LEARNED, **best))
+            dict(record_k=K, ray_begin=rb, seed=L.SEED_LEARNED, flags=group_flag(args), **best))
         bounces = int(cd[:, L.CTR["bounces"]].sum())
         ts, te = 0.0, dt
         _progress(f"full form: {steps} steps, {dt * 1e3 / steps:.1f} ms/step, {bounces / dt:.4g} bounces/s"
