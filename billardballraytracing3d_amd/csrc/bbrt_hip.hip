@@ -400,7 +400,8 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         bool have_h = false;
         PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
         const unsigned long long in_leaf = __ballot(r >= 0 && phase == kPhLeaf);
-        const int coop_max = (flags & BBRT_FLAG_STAGE1_WAVE) ? 64 : ((flags & BBRT_FLAG_STAGE1_LANE) ? 0 : kCoopLanes);
+        const int coop_max = (flags & BBRT_FLAG_STAGE1_WAVE) ? 64
+                             : ((flags & BBRT_FLAG_STAGE1_LANE) ? 0 : (LEARNED ? kCoopLanesLearned : kCoopLanes));
         if (COMPACT || (int)__popcll(in_leaf) > coop_max) {
             if (r >= 0 && phase == kPhLeaf) {
                 for (;;) {

@@ -70,7 +70,8 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
                                            are unchanged; kernel_ms[0] is then the whole graph's time */
        BBRT_FLAG_STAGE1_WAVE = 32u,     /* persistent kernel, stored scenes: every stage 1 wave-cooperative */
        BBRT_FLAG_STAGE1_LANE = 64u,     /* ... every stage 1 per lane (default: wave-cooperative when at most
-                                           8 lanes of a wave need one); results and counters do not change */
+                                           16 lanes of a wave need one, 8 with the learned seed); results and
+                                           counters do not change */
        BBRT_FLAG_GROUP_RAYS = 128u };   /* persistent kernel: take the live rays grouped by where they pass
                                            their scene (Morton order of 32^3 cells of the root box; sub-scene
                                            major) instead of in input order.  Each wave then traces rays that
