@@ -149,24 +149,42 @@ __global__ void __launch_bounds__(1024) k_init(const double* __restrict__ origin
 }
 
 // Exclusive scan of the group bins' counts: start[b] = rays in bins before b.
+// One block of 1024 threads, kScanChunk bins a pass: the chunk is loaded
+// coalesced into LDS, thread t scans its 8 consecutive bins, the 1024 sums
+// are scanned across the block, and the starts go back out coalesced; the
+// running total carries into the next pass.
+constexpr int kScanChunk = 8192;   // 1024 threads x 8 bins
 __global__ void __launch_bounds__(1024) k_scan_bins(const int32_t* __restrict__ hist, int32_t* __restrict__ start,
                                                     int nbins) {
+    __shared__ int buf[kScanChunk];
     __shared__ int part[1024];
     const int t = (int)threadIdx.x;
-    const int per = (nbins + 1023) / 1024;
-    const int b0 = min(nbins, t * per), b1 = min(nbins, b0 + per);
-    int sum = 0;
-    for (int b = b0; b < b1; ++b) sum += hist[b];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = t >= off ? part[t - off] : 0;
+    int carry = 0;
+    for (int base = 0; base < nbins; base += kScanChunk) {
+        const int m = min(kScanChunk, nbins - base);
+        for (int i = t; i < kScanChunk; i += 1024) buf[i] = i < m ? hist[base + i] : 0;
         __syncthreads();
-        part[t] += v;
+        int v[8];
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { v[k] = buf[t * 8 + k]; sum += v[k]; }
+        part[t] = sum;
         __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int u = t >= off ? part[t - off] : 0;
+            __syncthreads();
+            part[t] += u;
+            __syncthreads();
+        }
+        int run = carry + (t ? part[t - 1] : 0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { buf[t * 8 + k] = run; run += v[k]; }
+        const int total = part[1023];
+        __syncthreads();
+        for (int i = t; i < m; i += 1024) start[base + i] = buf[i];
+        carry += total;
+        __syncthreads();   // the next pass overwrites buf and part
     }
-    int run = t ? part[t - 1] : 0;
-    for (int b = b0; b < b1; ++b) { start[b] = run; run += hist[b]; }
 }
 
 // The live list grouped by bin: each ray to the next free place of its bin.
