@@ -192,7 +192,7 @@ def main():
                    "queue_order": (args.in_flight_order if in_flight > 1 else "input"),
                    "max_bounces": mb, "record_k": K, "newton_mode": args.mode, "seed_mode": "triangle",
                    "culling": not args.no_cull, "scene_seed": args.seed,
-                   "schedule": "wavefront" if args.wavefront else "persistent", "block": args.block or 256,
+                   "schedule": "wavefront" if args.wavefront else "persistent", "block": args.block or 64,
                    "waves_per_eu": args.waves_per_eu or 2, "newton_batch": args.newton_batch or 64},
         "roofline": {"bound": "mfma", "compute": "fp64 (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

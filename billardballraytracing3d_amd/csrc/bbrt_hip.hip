@@ -744,7 +744,10 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         if (out && out->ray_work) throw std::runtime_error("bbrt_trace: ray_work is not recorded with the learned seed");
     }
     if (n < 0 || n > (int64_t)INT32_MAX - 1024) throw std::runtime_error("bbrt_trace: ray count out of range");
-    const int block = p->block > 0 ? p->block : 256;
+    // default block: 64 threads for the stored-scene triangle-seed kernel (as fast as 256 one trace at a
+    // time, and with traces in flight the next trace's waves start as soon as one wave's slot frees);
+    // 256 for the learned seed (its 21 KB of LDS per block) and for compact scenes
+    const int block = p->block > 0 ? p->block : ((learned || s->compact) ? 256 : 64);
     if (block % 64 != 0 || block > 256) throw std::runtime_error("bbrt_trace: block must be a multiple of 64 <= 256");
     if (!st) st = s->stream;
     check(hipSetDevice(s->device), "hipSetDevice");

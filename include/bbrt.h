@@ -86,7 +86,8 @@ typedef struct {
     int32_t max_bounces;   /* run(max_bounces), RayTracingDriver.hpp:143 */
     int32_t record_k;      /* bounces recorded per ray in hit_patch / hit_xyz / newton_iters */
     uint32_t flags;        /* BBRT_FLAG_* */
-    int32_t block;         /* threads per block of the bounce kernel; 0 = library default */
+    int32_t block;         /* threads per block of the bounce kernel; 0 = library default (64 for the
+                              persistent kernel on a stored scene with the triangle seed, else 256) */
     int32_t waves_per_eu;  /* occupancy target of the persistent kernel: 0 = library default (2), or 2, 3, 4
                               (the compiler keeps registers within that occupancy, spilling if needed) */
     float grid_fraction;   /* persistent kernel: fraction of the resident workgroups launched (0 = all) */
