@@ -223,7 +223,7 @@ def main():
         res["roofline"]["traffic"] = traffic
         res["roofline"]["traffic_source"] = "profiles/k_persist_sinai_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)"
     if args.cpu_seconds > 0:
-        res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, mode, args.cpu_seconds, gs, K)
+        res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, args.cpu_seconds, gs, K)
 
     # the batched legs need the HBM the Sinai scene and its buffers hold
     del to, td, outs, ptrs, gs
@@ -465,7 +465,7 @@ def full_form_leg(args):
     hp = outs["hit_patch"].cpu().numpy(); hx = outs["hit_xyz"].cpu().numpy()
     rays_checked = mism = 0
     max_err = 0.0
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     for i in np.linspace(0, len(scenes) - 1, 6).astype(int):
         sl = slice(int(rb[i]), int(min(rb[i + 1], rb[i] + 600)))
         r = O.OracleScene(scenes[i], threads=threads).trace(o[sl], d[sl], mb, mode=L.NEWTON_CONVERGED, record_k=K,
@@ -606,14 +606,40 @@ def copy_probe(mb):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}), flush=True)
 
 
-def cpu_baseline(sc, o, d, mb, mode, budget_s, gs, K):
-    """The CPU oracle (the reference algorithm: brute-force stage 1, FP64, OpenMP
-    dynamic over rays) on consecutive 2,048-ray chunks of the same batch until
-    the time budget is spent; also checks the GPU's records for those rays."""
+def host_threads():
+    """Host threads for the CPU legs: OMP_NUM_THREADS when set (the process's share of the host's cores),
+    else every core this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "").strip()
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def host_cpu():
+    """nproc, the cores this process may use and the CPU model, as recorded beside the CPU baseline."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "model": model}
+
+
+def cpu_baseline(sc, o, d, mb, budget_s, gs, K):
+    """The CPU oracle timed as the reference runs: brute-force stage 1, the reference's Newton stopping rule
+    (step < 1e-3, SurfacePatch.hpp:629), FP64, OpenMP dynamic over rays (PlaneSourceDriver.hpp:96), on all the
+    host threads this process may use, over consecutive 2,048-ray chunks of the same batch until the time budget
+    is spent.  The GPU's records for those rays, traced in the same Newton mode, are checked against it."""
     import numpy as np
+    from billardballraytracing3d_amd import _lib as L
     from oracle import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
+    mode = L.NEWTON_REF
+    threads = host_threads()
     osc = O.OracleScene(sc, threads=threads)
     chunk = 2048
     t_tot = 0.0
@@ -638,8 +664,10 @@ def cpu_baseline(sc, o, d, mb, mode, budget_s, gs, K):
         start += chunk
     return {"value": bounces / t_tot if t_tot > 0 else None, "unit": "bounces/s", "cores": threads, "kind": "port",
             "sample": f"first {rays} rays of the same batch ({bounces} bounces) in {t_tot:.1f} s, oracle "
-                      f"brute-force stage 1, OpenMP dynamic",
-            "gpu_check": {"rays": rays, "max_rel_hit_error": max_err, "rays_with_sequence_mismatch": seq_mismatch}}
+                      f"brute-force stage 1, reference Newton rule (step < 1e-3), OpenMP dynamic",
+            "host": host_cpu(),
+            "gpu_check": {"rays": rays, "newton_mode": "ref", "max_rel_hit_error": max_err,
+                          "rays_with_sequence_mismatch": seq_mismatch}}
 
 
 if __name__ == "__main__":

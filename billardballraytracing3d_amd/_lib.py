@@ -36,7 +36,8 @@ EXPORTS = ("bbrt_scene_create", "bbrt_scene_destroy", "bbrt_scene_info", "bbrt_s
            "bbrt_mlp_forward", "bbrt_mlp_loss_grad", "bbrt_mlp_train", "bbrt_mlp_sample_pairs", "bbrt_mlp_features",
            "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab", "bbrt_scene_create_batch", "bbrt_scene_batch_info",
            "bbrt_trace_batch", "bbrt_trace_batch_device", "bbrt_autotune", "bbrt_scene_create_ex",
-           "bbrt_trace_device_async")
+           "bbrt_trace_device_async", "bbrt_trace_device_noalloc")
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
@@ -129,6 +130,8 @@ def lib():
                                               C.POINTER(Outputs), _i64, C.c_void_p, _d]
         L.bbrt_trace_device_async.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, _i64, C.POINTER(Params),
                                               C.POINTER(Outputs), C.c_void_p, C.c_void_p]
+        L.bbrt_trace_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Params),
+                                                C.POINTER(Outputs), C.c_void_p, C.c_void_p]
         L.bbrt_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.c_int64, C.POINTER(Params),
                                     C.POINTER(Outputs), C.POINTER(Params), C.POINTER(TuneEntry), C.c_int32, _i32,
                                     C.c_void_p]

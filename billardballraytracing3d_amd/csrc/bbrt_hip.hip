@@ -554,6 +554,10 @@ __global__ void k_zero_i32(int32_t* p, int n) {
 
 }  // namespace
 
+// Scratch of the no-allocation entry point, bbrt_trace_device_noalloc (file scope;
+// each GPU holds its own copy at its own address until the process ends).
+static __device__ __align__(256) unsigned char scratch[BBRT_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -716,11 +720,39 @@ __global__ void k_ray_scene(const int64_t* __restrict__ begin, int n_scenes, int
     out[g] = lo;
 }
 
+// The no-allocation entry point's scratch (bbrt_trace_device_noalloc): ray state,
+// live lists, bins, counts and counters of one trace, in a file-scope device array
+// that each GPU holds until the process ends.  Host code never passes `scratch`
+// itself to a kernel (in host code the name is a stand-in, not a device address):
+// noalloc_scratch() takes the array's address on the current device with
+// hipGetSymbolAddress once per device and keeps it.
+constexpr size_t kNoAllocScratchBytes = BBRT_NOALLOC_SCRATCH_BYTES;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+char* g_noalloc_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current device.
+char* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("bbrt_trace_device_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("bbrt_trace_device_noalloc: no device address for the scratch array");
+        g_noalloc_ptr[device] = static_cast<char*>(p);
+    }
+    return g_noalloc_ptr[device];
+}
+
 // async: enqueue on st and return without waiting (bbrt_trace_device_async):
 // the stream's own scratch, counters (if d_counters) written on the device.
+// fixed_scratch (with async): a caller-owned scratch of kNoAllocScratchBytes
+// instead of the handle's; nothing is allocated.
 void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, const bbrt_params* p,
                 const bbrt_outputs* out, int64_t* counters, hipStream_t st, double* kernel_ms, bool pad_records,
-                const int32_t* d_ray_scene, bool async = false, int64_t* d_counters = nullptr) {
+                const int32_t* d_ray_scene, bool async = false, int64_t* d_counters = nullptr,
+                char* fixed_scratch = nullptr) {
     if (!p) throw std::runtime_error("bbrt_trace: params is NULL");
     if (async && (p->flags & (BBRT_FLAG_WAVEFRONT | BBRT_FLAG_GRAPH | BBRT_FLAG_TIME_KERNELS)))
         throw std::runtime_error("bbrt_trace_device_async: BBRT_FLAG_WAVEFRONT, BBRT_FLAG_GRAPH and "
@@ -744,10 +776,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         if (out && out->ray_work) throw std::runtime_error("bbrt_trace: ray_work is not recorded with the learned seed");
     }
     if (n < 0 || n > (int64_t)INT32_MAX - 1024) throw std::runtime_error("bbrt_trace: ray count out of range");
-    // default block: 64 threads for the stored-scene triangle-seed kernel (as fast as 256 one trace at a
-    // time, and with traces in flight the next trace's waves start as soon as one wave's slot frees);
-    // 256 for the learned seed (its 21 KB of LDS per block) and for compact scenes
-    const int block = p->block > 0 ? p->block : ((learned || s->compact) ? 256 : 64);
+    const int block = p->block > 0 ? p->block : default_block(learned, s->compact != 0);
     if (block % 64 != 0 || block > 256) throw std::runtime_error("bbrt_trace: block must be a multiple of 64 <= 256");
     if (!st) st = s->stream;
     check(hipSetDevice(s->device), "hipSetDevice");
@@ -781,7 +810,12 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     const size_t off_cnt = align256(off_cur + (size_t)nbins * 4);
     const size_t off_ctr = align256(off_cnt + 256);
     const size_t need = align256(off_ctr + BBRT_NCOUNTERS * 8);
-    char* base = static_cast<char*>(async ? async_scratch_for(s, st, need) : scratch_for(s, need));
+    if (fixed_scratch && need > kNoAllocScratchBytes)
+        throw std::runtime_error("bbrt_trace_device_noalloc: " + std::to_string(n) + " rays need " +
+                                 std::to_string(need) + " bytes of scratch, more than the " +
+                                 std::to_string(kNoAllocScratchBytes) + " of the fixed scratch array");
+    char* base = fixed_scratch ? fixed_scratch
+                               : static_cast<char*>(async ? async_scratch_for(s, st, need) : scratch_for(s, need));
     RayState R{ reinterpret_cast<double*>(base + off_o), reinterpret_cast<double*>(base + off_d) };
     int32_t* live[2] = { reinterpret_cast<int32_t*>(base + off_la), reinterpret_cast<int32_t*>(base + off_lb) };
     int32_t* cnt = reinterpret_cast<int32_t*>(base + off_cnt);   // [0], [1]: list sizes, [2]: queue head
@@ -1417,6 +1451,24 @@ int bbrt_trace_device_async(bbrt_scene s, const double* origins, const double* d
     }
 }
 
+int bbrt_trace_device_noalloc(bbrt_scene s, const double* origins, const double* dirs, int64_t n,
+                              const bbrt_params* p, const bbrt_outputs* out, int64_t* d_counters, void* stream) {
+    if (!s) { g_last_error = "bbrt_trace_device_noalloc: NULL scene"; return -1; }
+    try {
+        if (s->n_scenes > 1)
+            throw std::runtime_error("bbrt_trace_device_noalloc: one-scene handles only (a batch handle's per-ray "
+                                     "sub-scene ids would need device memory)");
+        std::lock_guard<std::mutex> lock(s->async_mu);
+        check(hipSetDevice(s->device), "hipSetDevice");
+        char* base = noalloc_scratch(s->device);
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+        trace_impl(s, origins, dirs, n, p, out, nullptr, st, nullptr, false, nullptr, true, d_counters, base);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 int bbrt_trace(bbrt_scene s, const double* origins, const double* dirs, int64_t n, const bbrt_params* p,
                const bbrt_outputs* out, int64_t* counters) {
     return trace_host(s, origins, dirs, n, nullptr, p, out, counters, "bbrt_trace");
@@ -1474,7 +1526,8 @@ int bbrt_autotune(bbrt_scene s, const double* origins, const double* dirs, const
         // coordinate descent from the library defaults over occupancy, Newton batch, block size and grid size
         bbrt_params cur = *base;
         cur.flags &= ~(uint32_t)BBRT_FLAG_GRAPH;
-        cur.block = 256; cur.waves_per_eu = default_waves_per_eu(s->compact); cur.newton_batch = kDefaultNewtonBatch;
+        cur.block = default_block(base->seed_mode == BBRT_SEED_LEARNED, s->compact != 0);
+        cur.waves_per_eu = default_waves_per_eu(s->compact); cur.newton_batch = kDefaultNewtonBatch;
         cur.grid_fraction = 1.0f;
         double cur_ms = run(cur);
         auto sweep = [&](auto setter, std::initializer_list<double> values) {
@@ -1490,7 +1543,7 @@ int bbrt_autotune(bbrt_scene s, const double* origins, const double* dirs, const
         };
         sweep([](bbrt_params& q, double v) { q.waves_per_eu = (int)v; }, { 2, 3, 4 });
         sweep([](bbrt_params& q, double v) { q.newton_batch = (int)v; }, { 8, 16, 32, 64 });
-        sweep([](bbrt_params& q, double v) { q.block = (int)v; }, { 128, 256 });
+        sweep([](bbrt_params& q, double v) { q.block = (int)v; }, { 64, 128, 256 });
         sweep([](bbrt_params& q, double v) { q.grid_fraction = (float)v; }, { 0.5, 0.75, 1.0 });
         *best = cur;
         best->flags = base->flags;

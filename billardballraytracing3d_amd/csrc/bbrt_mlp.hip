@@ -974,8 +974,13 @@ int bbrt_mlp_features(bbrt_scene s, const int32_t* patch, const double* origin, 
 int bbrt_scene_set_mlp(bbrt_scene s, bbrt_mlp m) {
     if (!s || !m) { g_last_error = "bbrt_scene_set_mlp: NULL argument"; return -1; }
     try {
+        // No asynchronous trace may be enqueued meanwhile, and every trace already in flight on the device
+        // (bbrt_trace_device_async / _noalloc, on any stream) ends before the network buffers are rewritten:
+        // it reads them through the whole launch.
+        std::lock_guard<std::mutex> lock(s->async_mu);
         check(hipSetDevice(s->device), "hipSetDevice");
         if (m->device != s->device) throw std::runtime_error("bbrt_scene_set_mlp: network and scene on different devices");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         ensure_frames(s);
         check(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
         if (!s->mlp_h1pre) {

@@ -22,6 +22,10 @@ constexpr int kDefaultWavesPerEu = 2;         // persistent-kernel occupancy tar
                                               // kernel spill-free and measured 2-4 % faster than 3 on MI355X
 constexpr int kDefaultWavesPerEuCompact = 2;  // compact scenes: the on-the-fly cell evaluation wants the registers
 inline int default_waves_per_eu(bool compact) { return compact ? kDefaultWavesPerEuCompact : kDefaultWavesPerEu; }
+// threads per block of the persistent kernel: 64 for stored scenes with the triangle seed (as fast as 256
+// one trace at a time; with traces in flight the next trace's waves start as soon as one wave's slot frees),
+// 256 for the learned seed (21 KB of LDS per block) and for compact scenes
+inline int default_block(bool learned, bool compact) { return (learned || compact) ? 256 : 64; }
 constexpr int kDefaultNewtonBatch = 64;       // parked lanes (stage-1 hits) that trigger a batched Newton (64: when no lane can go on)
 constexpr double kMtEps = 1e-9;               // MollerTrumbore.hpp:33
 constexpr double kOnNodeTol = 1.0e-12;        // PatchInterpolation.hpp:21

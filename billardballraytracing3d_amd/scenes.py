@@ -239,11 +239,12 @@ class Obstacle:
     rot_seed: int
     node_kind: str = "lobatto"   # tensor node set: "lobatto" or "cheb1"
     n_nodes: int = 0             # nodes per patch side; 0 means order + 1
+    rotate: bool = True          # False: the polyhedron in its standard orientation (octahedron vertices on the axes)
 
 
 def obstacle_patches(ob: Obstacle) -> dict:
     """Tensor patches of one obstacle: dict of nodes, fields (T, 12, n*n) and triangle nodes."""
-    rot = random_rotation(SplitMix64(ob.rot_seed))
+    rot = random_rotation(SplitMix64(ob.rot_seed)) if ob.rotate else None
     corners = sphere_triangles(ob.mesh, ob.freq, rot)
     p = ob.order
     tn = triangle_nodes(p, corners)                       # (T,K,3) on unit sphere

@@ -214,6 +214,21 @@ class GpuScene:
                                              C.c_void_p(stream) if stream else None)
         L.check(rc, "bbrt_trace_device_async")
 
+    def trace_device_noalloc(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
+                             counters_ptr: int = 0, mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0,
+                             stream: int = 0, block: int = 0, waves_per_eu: int = 0, seed: int = L.SEED_TRIANGLE,
+                             newton_batch: int = 0, grid_fraction: float = 0.0):
+        """bbrt_trace_device_noalloc: bbrt_trace_device_async with the trace's scratch in the library's fixed
+        per-GPU array (L.NOALLOC_SCRATCH_BYTES) instead of device memory allocated for it.  Enqueues on `stream`
+        and returns; calls that overlap on one GPU share that array and race."""
+        p = _params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newton_batch, grid_fraction)
+        out = _outputs(out_ptrs)
+        rc = L.lib().bbrt_trace_device_noalloc(self.h, C.c_void_p(origins_ptr), C.c_void_p(dirs_ptr), C.c_int64(n),
+                                               C.byref(p), C.byref(out),
+                                               C.c_void_p(counters_ptr) if counters_ptr else None,
+                                               C.c_void_p(stream) if stream else None)
+        L.check(rc, "bbrt_trace_device_noalloc")
+
     def autotune(self, origins_ptr: int, dirs_ptr: int, n: int, max_bounces: int, out_ptrs: dict,
                  mode: int = L.NEWTON_CONVERGED, record_k: int = 0, flags: int = 0, stream: int = 0,
                  seed: int = L.SEED_TRIANGLE, ray_begin=None):

@@ -226,11 +226,32 @@ int bbrt_trace_device_async(bbrt_scene scene, const double* origins, const doubl
                             const int64_t* ray_begin, const bbrt_params* params, const bbrt_outputs* out,
                             int64_t* d_counters, void* stream);
 
+/* bbrt_trace_device_async that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The trace's scratch (ray state, live lists, queue bins, counts, counters: 56
+ * bytes per ray, 60 with BBRT_FLAG_GROUP_RAYS, plus up to 256 KB) lives in one
+ * file-scope device array of BBRT_NOALLOC_SCRATCH_BYTES per GPU, which each GPU
+ * the library runs on holds until the process ends; traces needing more are
+ * refused with an error (2^20 rays fit in every mode).  The call clears what
+ * must start at zero, enqueues the trace on `stream` (NULL = the scene's own
+ * stream) and returns; counters go to d_counters (device memory, may be NULL).
+ * Every call on one GPU shares that GPU's array: calls whose traces overlap on
+ * one GPU race (enqueue them on one stream, or wait between them).  One-scene
+ * handles only; BBRT_FLAG_WAVEFRONT, BBRT_FLAG_GRAPH and BBRT_FLAG_TIME_KERNELS
+ * are refused, as for bbrt_trace_device_async.  Results and counters equal
+ * bbrt_trace_device's. */
+#define BBRT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int bbrt_trace_device_noalloc(bbrt_scene scene, const double* origins, const double* dirs, int64_t n,
+                              const bbrt_params* params, const bbrt_outputs* out, int64_t* d_counters, void* stream);
+
 /* Launch-shape autotuner for the persistent kernel.  Times the trace of the
  * given device-resident rays (ray_begin as for bbrt_trace_batch, or NULL and n
  * rays of a single scene; outputs in `out` are overwritten) under a coordinate
- * descent from the library defaults: waves_per_eu {2, 3, 4}, newton_batch
- * {8, 16, 32, 64}, block {128, 256}, grid_fraction {0.5, 0.75, 1}; each
+ * descent from the library defaults of the kernel the trace selects (block 64
+ * for a stored scene with the triangle seed, else 256; 2 waves/EU; Newton
+ * batch 64; the full grid): waves_per_eu {2, 3, 4}, newton_batch
+ * {8, 16, 32, 64}, block {64, 128, 256}, grid_fraction {0.5, 0.75, 1}; each
  * candidate is the faster of two runs.  *best = *base with the fastest shape;
  * table (may be NULL) receives up to max_entries timed candidates in order. */
 typedef struct {
@@ -326,7 +347,11 @@ int bbrt_mlp_sample_pairs(bbrt_scene scene, uint64_t seed, int64_t first, int64_
 int bbrt_mlp_features(bbrt_scene scene, const int32_t* patch, const double* origin, const double* dir, int64_t n,
                       float* x, double* base, double* frame);
 /* Attaches a snapshot of the network to the scene (per-patch frames and the
- * patch half of layer 1 are precomputed); BBRT_SEED_LEARNED then uses it. */
+ * patch half of layer 1 are precomputed); BBRT_SEED_LEARNED then uses it.
+ * Waits first for all work on the scene's device, so traces already enqueued
+ * (bbrt_trace_device_async, bbrt_trace_device_noalloc, on any stream) finish
+ * with the network they were enqueued with; traces enqueued after the call
+ * returns use the new one. */
 int bbrt_scene_set_mlp(bbrt_scene scene, bbrt_mlp mlp);
 /* Seed A/B on sampled pairs (indices first .. first+n-1, same draw as
  * training): for each pair, stage 1, then converged Newton from the triangle

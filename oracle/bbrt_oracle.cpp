@@ -19,8 +19,16 @@
 //       that boundary: MKL is not in this image);
 //   cell padding ... paraboloid closed form (sagitta = |a+b| (1/99)^2);
 //   Newton ... paraboloid closed form, forced singular / no-convergence /
-//       domain events;
-//   bounce loop ... reflection invariants, head-on sphere, periodic orbit.
+//       domain events (tests/test_oracle_math.py);
+//   stage-1 winner and triangle_uv_centroid seed ... the coarse triangle
+//       under a flat patch's hit point and the mean of its corner (u, v);
+//       Newton from exactly that seed (tests/test_oracle_bounce.py);
+//   bounce loop ... parallel flat plates (exact reflected directions, hit
+//       points, bounce count, nudge 1e-9 * diagonal), reflection invariants
+//       and the nudge 5 * max sagitta of the hit patch on paraboloid patches,
+//       zero direction, immediate escape, head-on sphere, the periodic x-axis
+//       orbit between two spheres (bit-exact, Trapped) and its divergence
+//       rate 2 + sqrt(3) (tests/test_oracle_bounce.py).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -402,14 +410,20 @@ bool stage1(const Patch& P, const Vec3& o, const Vec3& d, double& bt, int& bi, i
     return have;
 }
 
+// triangle_uv_centroid (SurfacePatch.hpp:499-510): the (u, v) seed of the stage-1
+// winner, triangle (p00, p10, p11) or, when `second`, (p00, p11, p01).
+void triangle_uv_centroid(const Patch& P, int i, int j, bool second, double& u0, double& v0) {
+    const double ui = P.cu[i], uip1 = P.cu[i + 1], vj = P.cv[j], vjp1 = P.cv[j + 1];
+    if (!second) { u0 = (ui + uip1 + uip1) / 3.0; v0 = (vj + vj + vjp1) / 3.0; }
+    else { u0 = (ui + ui + uip1) / 3.0; v0 = (vj + vjp1 + vjp1) / 3.0; }
+}
+
 // SurfacePatch.hpp:253-303 (+ triangle_uv_centroid, :499-510)
 PatchHit patch_intersect(const Patch& P, const Vec3& o, const Vec3& d, int mode, Counters& C) {
     double bt = 0; int bi = -1, bj = -1; bool bsec = false;
     if (!stage1(P, o, d, bt, bi, bj, bsec, C)) return PatchHit{ 0, 0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
-    const double ui = P.cu[bi], uip1 = P.cu[bi + 1], vj = P.cv[bj], vjp1 = P.cv[bj + 1];
     double u0, v0;
-    if (!bsec) { u0 = (ui + uip1 + uip1) / 3.0; v0 = (vj + vj + vjp1) / 3.0; }
-    else { u0 = (ui + ui + uip1) / 3.0; v0 = (vj + vjp1 + vjp1) / 3.0; }
+    triangle_uv_centroid(P, bi, bj, bsec, u0, v0);
     return newton_refine(P, o, d, u0, v0, bt, mode, C);
 }
 
@@ -768,6 +782,20 @@ int orc_patch_intersect_uv(void* s, int64_t k, const double* o, const double* d,
     out[4] = h.normal.x; out[5] = h.normal.y; out[6] = h.normal.z; out[7] = h.u; out[8] = h.v;
     *iters = h.iters;
     return h.status;
+}
+
+// Stage 1 alone and the Newton seed it gives (SurfacePatch.hpp:272-302): returns 1 on a
+// stage-1 hit with out = t0, u0, v0 and idx = i, j, second triangle; 0 on a miss.
+int orc_stage1_seed(void* s, int64_t k, const double* o, const double* d, double* out, int* idx) {
+    Counters C;
+    const Patch& P = static_cast<Scene*>(s)->patches[k];
+    double bt = 0; int bi = -1, bj = -1; bool bsec = false;
+    if (!stage1(P, { o[0], o[1], o[2] }, { d[0], d[1], d[2] }, bt, bi, bj, bsec, C)) return 0;
+    double u0, v0;
+    triangle_uv_centroid(P, bi, bj, bsec, u0, v0);
+    out[0] = bt; out[1] = u0; out[2] = v0;
+    idx[0] = bi; idx[1] = bj; idx[2] = bsec ? 1 : 0;
+    return 1;
 }
 
 // Newton alone from a given seed (SurfacePatch::newton_refine).

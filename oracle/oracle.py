@@ -59,6 +59,8 @@ def lib():
         L.orc_patch_intersect.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_int, _d, C.POINTER(C.c_int)]
         L.orc_patch_intersect_uv.restype = C.c_int
         L.orc_patch_intersect_uv.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_int, _d, C.POINTER(C.c_int)]
+        L.orc_stage1_seed.restype = C.c_int
+        L.orc_stage1_seed.argtypes = [C.c_void_p, C.c_int64, _d, _d, _d, C.POINTER(C.c_int)]
         L.orc_newton.restype = C.c_int
         L.orc_newton.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int, _d,
                                  C.POINTER(C.c_int)]
@@ -229,6 +231,14 @@ class OracleScene:
         st = lib().orc_patch_intersect_uv(self.h, C.c_int64(k), ptr(_v(o), _d), ptr(_v(d), _d), C.c_int(mode),
                                           ptr(out, _d), C.byref(it))
         return st, out[0], out[1:4], out[4:7], it.value, out[7], out[8]
+
+    def stage1_seed(self, k, o, d):
+        """Stage 1 of SurfacePatch::intersect alone: None on a miss, else (t0, u0, v0, i, j, second)."""
+        out = np.zeros(3); idx = (C.c_int * 3)()
+        h = lib().orc_stage1_seed(self.h, C.c_int64(k), ptr(_v(o), _d), ptr(_v(d), _d), ptr(out, _d), idx)
+        if not h:
+            return None
+        return out[0], out[1], out[2], idx[0], idx[1], bool(idx[2])
 
     def newton(self, k, o, d, u0, v0, t0, mode=0):
         out = np.zeros(7); it = C.c_int(0)

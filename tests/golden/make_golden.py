@@ -8,7 +8,8 @@ reference's outputs only:
 
   mt_cases.npz      Moller-Trumbore (MollerTrumbore.hpp:32-64): rays, triangles, hit flag, t, point, normal
   raybox_cases.npz  ray_box (BVH.hpp:14-37) incl. zero direction components (0 * inf = NaN) and best_t prunes
-  bvh_cases.npz     BVH(objects, threshold).intersect (BVH.hpp:65-189) over probe objects: visit order, winner
+  bvh_cases.npz     BVH(objects, threshold).intersect (BVH.hpp:65-189) over probe objects: visit order, winner;
+                    7 of 8 rays aimed near an object's centroid, so most walks reach several leaves
   kat.npz           the known-answer scene of tests/Triangle_Intersect.cpp:14-31 (BVH and brute force t)
 
 Usage: python tests/golden/make_golden.py
@@ -108,9 +109,16 @@ def bvh_cases(rng):
         if trial % 7 == 0:
             t[:] = 5.0               # tied hits
         o = rng.uniform(-8, 8, 3)
-        d = rng.normal(size=3)
-        if trial % 3 == 0:
-            d[int(rng.integers(0, 3))] = 0.0
+        if trial % 8 == 7:           # a random direction
+            d = rng.normal(size=3)
+        else:                        # aimed at a point near one object's centroid, so the walk reaches leaves
+            tgt = c[int(rng.integers(0, n))] + rng.uniform(-0.5, 0.5, 3) * h[0]
+            d = tgt - o
+        if trial % 3 == 0:           # a zero direction component (0 * inf = NaN in the slab tests)
+            k = int(rng.integers(0, 3))
+            if trial % 8 != 7:
+                o[k] = tgt[k]        # keeps the aimed ray on its target
+            d[k] = 0.0
         d /= np.linalg.norm(d)
         lt = int(rng.integers(1, 6))
         vis, best, bt = O.bvh_visit(boxes, c, t, o, d, lt, fn="ref_bvh_visit")

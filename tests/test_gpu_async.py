@@ -170,3 +170,51 @@ def test_grouped_queue_changes_nothing(sinai):
     for k in ref:
         assert torch.equal(got[k], ref[k]), k
     np.testing.assert_array_equal(c_got[:10], c_ref[:10])
+
+
+def test_set_mlp_waits_for_traces_in_flight(sinai):
+    """bbrt_scene_set_mlp while a learned-seed trace is still running on another stream: the trace finishes
+    with the network it was enqueued with (records, Newton iterations and learned/fallback counts equal a
+    synchronous trace with that network), and traces enqueued afterwards use the new one."""
+    sc, o, d, mb, gs0 = sinai
+    gs = T.GpuScene(sc)
+    n, K = o.shape[0], 12
+    to, td = torch.from_numpy(o).to(DEV), torch.from_numpy(d).to(DEV)
+    net_a = T.Mlp(5)                      # an untrained network: some seeds kept, many fall back
+    net_b = T.Mlp(6)
+    rng = np.random.default_rng(7)
+    pb = net_b.params
+    net_b.params = np.where(pb != 0, rng.standard_normal(pb.size).astype(np.float32) * 3.0, 0.0).astype(np.float32)
+
+    def outs():
+        o_ = _outs(n, K)
+        o_["seed_stats"] = torch.zeros((n, 2), dtype=torch.int32, device=DEV)
+        return o_
+
+    refs = {}
+    for name, net in (("a", net_a), ("b", net_b)):
+        gs.set_mlp(net)
+        r = outs()
+        c, _ = gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(r), record_k=K, seed=L.SEED_LEARNED)
+        refs[name] = (r, c)
+    torch.cuda.synchronize()
+    # the two networks really differ in what they do
+    assert not torch.equal(refs["a"][0]["seed_stats"], refs["b"][0]["seed_stats"])
+    gs.set_mlp(net_a)
+    st = torch.cuda.Stream(DEV)
+    got_a, got_b = outs(), outs()
+    ctr = torch.zeros((2, L.NCOUNTERS), dtype=torch.int64, device=DEV)
+    torch.cuda.synchronize()
+    gs.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(got_a), counters_ptr=ctr[0].data_ptr(),
+                          record_k=K, seed=L.SEED_LEARNED, stream=st.cuda_stream)
+    gs.set_mlp(net_b)                     # while the trace above may still run
+    gs.trace_device_async(to.data_ptr(), td.data_ptr(), n, mb, _ptrs(got_b), counters_ptr=ctr[1].data_ptr(),
+                          record_k=K, seed=L.SEED_LEARNED, stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    sched = (L.CTR["launches"], L.CTR["mlp_evals"], L.CTR["mlp_lanes"])
+    keep = [q for q in range(L.NCOUNTERS) if q not in sched]
+    for j, (got, name) in enumerate(((got_a, "a"), (got_b, "b"))):
+        ref, c_ref = refs[name]
+        for k in ref:
+            assert torch.equal(got[k], ref[k]), (name, k)
+        np.testing.assert_array_equal(ctr[j].cpu().numpy()[keep], c_ref[keep], err_msg=name)

@@ -110,3 +110,12 @@ def test_bvh_matches_reference_harness():
         a = O.bvh_visit(box, cent, t, o, d, 4)
         b = O.bvh_visit(box, cent, t, o, d, 4, fn="ref_bvh_visit")
         np.testing.assert_array_equal(a[0], b[0])
+
+
+def test_noalloc_scratch_size_matches_header():
+    """The Python binding's limit for bbrt_trace_device_noalloc is the header's BBRT_NOALLOC_SCRATCH_BYTES."""
+    txt = open(os.path.join(ROOT, "include", "bbrt.h")).read()
+    m = re.search(r"#define BBRT_NOALLOC_SCRATCH_BYTES \((\d+)u << (\d+)\)", txt)
+    assert m and int(m.group(1)) << int(m.group(2)) == L.NOALLOC_SCRATCH_BYTES
+    # 2^20 rays fit with the grouped queue: 60 bytes per ray + 2 * 32768 bins + alignment
+    assert 60 * (1 << 20) + 8 * 32768 + 4096 <= L.NOALLOC_SCRATCH_BYTES
