@@ -68,10 +68,16 @@ def main():
     ap.add_argument("--batch-in-flight", type=int, default=2,
                     help="batched leg: traces kept in flight (bbrt_trace_device_async); 1 = graph replays only")
     ap.add_argument("--batch-no-tune", action="store_true", help="batched leg: library default launch shape")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="batched leg: eager launches instead of hipGraph replays (for counter profiles)")
+    ap.add_argument("--batch-no-check", dest="batch_check", action="store_false",
+                    help="batched leg: skip the CPU oracle check of sampled scenes")
     ap.add_argument("--batch-only", action="store_true", help="run the batched-scenes leg alone")
     ap.add_argument("--workload", choices=["sinai", "full-form"], default="sinai",
                     help="full-form: the full batched form (512 scenes, 16M rays, 200 bounces, learned seeds) alone")
     ap.add_argument("--no-full-form", action="store_true", help="skip the full batched form leg (512 scenes, 16M rays)")
+    ap.add_argument("--full-no-triangle", dest="full_triangle", action="store_false",
+                    help="full form: skip the triangle-seed run of the same workload")
     ap.add_argument("--full-scenes", type=int, default=512)
     ap.add_argument("--full-rays", type=int, default=16_000_000)
     ap.add_argument("--full-in-flight", type=int, default=2,
@@ -199,7 +205,12 @@ def main():
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_bounce" if args.wavefront else "k_persist", "launches_per_step": launches / args.steps,
                      "avg_launch_ms": kernel_ms / launches if launches else None,
-                     "flops_per_launch": flops / launches if launches else None},
+                     "flops_per_launch": flops / launches if launches else None,
+                     "pass": "one synchronous trace at a time (each launch's own duration, HIP events)",
+                     "per_headline_step": {
+                         "achieved": (flops / launches) / (ms_per_step * 1e-3) / 1e12 if launches else None,
+                         "frac": (flops / launches) / (ms_per_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS if launches else None,
+                         "note": "the same flops per trace over the headline's ms_per_step (traces overlap in flight)"}},
         "one_in_flight": one_in_flight,
         "in_flight_check": in_flight_check,
         "mean_newton_iters": {"triangle": newton_iters / newton_calls if newton_calls else None, "learned": None},
@@ -218,10 +229,21 @@ def main():
                                          one_in_flight["value"], res)
             res["mean_newton_iters"]["learned"] = res["learned"]["mean_newton_iters"]
 
-    traffic = committed_traffic(args, n, mb, K)
+    traffic = committed_profile_value(args, n, mb, K, "traffic_bytes_per_launch")
+    avg_ms = kernel_ms / launches if launches else None
     if traffic is not None:
         res["roofline"]["traffic"] = traffic
         res["roofline"]["traffic_source"] = "profiles/k_persist_sinai_1M.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)"
+        if avg_ms:
+            res["roofline"]["hbm"] = {"achieved": traffic / (avg_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    lane_flops = committed_profile_value(args, n, mb, K, "fp64_valu_lane_flops_per_launch")
+    if lane_flops is not None and avg_ms:
+        res["roofline"]["measured_fp64_valu"] = {
+            "lane_flops_per_launch": lane_flops, "achieved": lane_flops / (avg_ms * 1e-3) / 1e12,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": lane_flops / (avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "source": "profiles/k_persist_sinai_1M.md (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes; issued lane "
+                      "slots, masked lanes included)"}
     if args.cpu_seconds > 0:
         res["cpu_baseline"] = cpu_baseline(sc, o, d, mb, args.cpu_seconds, gs, K)
 
@@ -253,6 +275,7 @@ def batch_leg(args):
     gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
     t2 = time.time()
     n_patches = gb.n_patches
+    sampled = {int(i): scenes[int(i)] for i in np.linspace(0, len(scenes) - 1, 6).astype(int)}
     del scenes
     n = int(rb[-1])
     mb, K = 50, 20
@@ -272,9 +295,11 @@ def batch_leg(args):
         best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st)
     t_tune = time.time() - ta
 
+    graph_flag = 0 if args.batch_eager else L.FLAG_GRAPH
+
     def step():
         return gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
-                               flags=L.FLAG_GRAPH, **best)
+                               flags=graph_flag | (L.FLAG_TIME_KERNELS if args.batch_eager else 0), **best)
 
     for _ in range(max(1, args.warmup)):
         step()
@@ -292,7 +317,8 @@ def batch_leg(args):
     flops = float(ctr[L.CTR["newton_flops"]]) + 27.0 * bounces
     graph_ms = kms[0] / args.batch_steps
     one_in_flight = {"value": bounces / (te - ts), "ms_per_step": (te - ts) * 1e3 / args.batch_steps,
-                     "note": "one hipGraph-replayed trace at a time"}
+                     "note": "one eager trace at a time" if args.batch_eager else "one hipGraph-replayed trace at a time"}
+    check = oracle_check(sampled, o, d, rb, outs, mb, K, "batch") if args.batch_check else None
     in_flight_check = None
     if args.batch_in_flight > 1:
         dt, cd, in_flight_check = run_in_flight(
@@ -308,7 +334,7 @@ def batch_leg(args):
         "newton_mode": "converged", "scene": "compact (coarse cells evaluated on the fly), device build",
         "bounces_per_step": bounces // args.batch_steps, "graph_ms_per_step": graph_ms,
         "fp64_tflops": flops / (graph_ms * args.batch_steps * 1e-3) / 1e12 if graph_ms > 0 else None,
-        "autotuned": best, "autotune_candidates": rows, "autotune_s": t_tune,
+        "autotuned": best, "autotune_candidates": rows, "autotune_s": t_tune, "oracle_check": check,
         "scene_generation_s": t1 - t0, "scene_build_s": t2 - t1,
         "counters_per_step": {k: int(ctr[v] // args.batch_steps) for k, v in L.CTR.items()},
     }
@@ -363,6 +389,41 @@ def run_in_flight(g, to, td, n, mb, outs, steps, warmup, in_flight, ref_counters
     return dt, cd, check
 
 
+def oracle_check(sampled, o, d, rb, outs, mb, K, who, rays_per_scene=600):
+    """The batch trace's records of the first rays_per_scene rays of each sampled sub-scene ({index: scene})
+    against the CPU oracle (FP64, triangle seed, converged Newton): status, error kind, bounce count and the
+    recorded patch sequence must be equal, and the hit points within 1e-10 relative."""
+    import numpy as np
+    from billardballraytracing3d_amd import _lib as L
+    from oracle import oracle as O
+
+    status = outs["status"].cpu().numpy(); bnc = outs["bounces"].cpu().numpy()
+    ek = outs["err_kind"].cpu().numpy()
+    hp = outs["hit_patch"].cpu().numpy(); hx = outs["hit_xyz"].cpu().numpy()
+    rays_checked = mism = 0
+    max_err = 0.0
+    threads = host_threads()
+    for i, sc in sampled.items():
+        sl = slice(int(rb[i]), int(min(rb[i + 1], rb[i] + rays_per_scene)))
+        r = O.OracleScene(sc, threads=threads).trace(o[sl], d[sl], mb, mode=L.NEWTON_CONVERGED, record_k=K,
+                                                     threads=threads)
+        kk = np.minimum(r.bounces, K)
+        m = np.arange(K)[None, :] < kk[:, None]
+        bad = (status[sl] != r.status) | (ek[sl] != r.err_kind) | (bnc[sl] != r.bounces) | \
+            np.any(np.where(m, hp[sl] != r.hit_patch, False), 1)
+        mism += int(bad.sum())
+        rays_checked += r.status.size
+        scale = np.maximum(np.linalg.norm(r.hit_xyz, axis=-1), 1.0)
+        err = np.where(m, np.linalg.norm(hx[sl] - r.hit_xyz, axis=-1) / scale, 0.0)
+        max_err = max(max_err, float(err.max()))
+        _progress(f"{who}: oracle check scene {i}: {r.status.size} rays, {mism} mismatches so far")
+    if mism or max_err > 1e-10:
+        raise SystemExit(f"{who}: records differ from the CPU oracle ({mism} rays, max hit error {max_err:.3g})")
+    return {"scenes": sorted(sampled), "rays": rays_checked, "rays_with_mismatch": mism,
+            "max_rel_hit_error": max_err, "bounces_compared": K,
+            "oracle": "CPU FP64, triangle seed, converged Newton"}
+
+
 def error_counts(err_kind):
     """Rays of the last step per error kind (device int8 array)."""
     import torch
@@ -393,6 +454,7 @@ def full_form_leg(args):
     dev = torch.device("cuda", 0)
     t0 = time.time()
     scenes, o, d, rb = S.batch_config(args.seed, n_scenes=args.full_scenes, total_rays=args.full_rays)
+    check_ids = np.linspace(0, len(scenes) - 1, 6).astype(int)
     t1 = time.time()
     _progress(f"full form: {len(scenes)} scenes, {int(rb[-1])} rays generated in {t1 - t0:.1f} s")
     gb = T.GpuScene.batch(scenes, compact=True, device_build=True)
@@ -461,25 +523,11 @@ def full_form_leg(args):
             per.append(a[1] / a.sum())
     per = np.array(per)
     # oracle check on sampled scenes: same rays, triangle seed, converged Newton, FP64
-    status = outs["status"].cpu().numpy(); bnc = outs["bounces"].cpu().numpy()
-    hp = outs["hit_patch"].cpu().numpy(); hx = outs["hit_xyz"].cpu().numpy()
-    rays_checked = mism = 0
-    max_err = 0.0
-    threads = host_threads()
-    for i in np.linspace(0, len(scenes) - 1, 6).astype(int):
-        sl = slice(int(rb[i]), int(min(rb[i + 1], rb[i] + 600)))
-        r = O.OracleScene(scenes[i], threads=threads).trace(o[sl], d[sl], mb, mode=L.NEWTON_CONVERGED, record_k=K,
-                                                            threads=threads)
-        kk = np.minimum(r.bounces, K)
-        m = np.arange(K)[None, :] < kk[:, None]
-        bad = (status[sl] != r.status) | (bnc[sl] != r.bounces) | np.any(np.where(m, hp[sl] != r.hit_patch, False), 1)
-        mism += int(bad.sum())
-        rays_checked += r.status.size
-        scale = np.maximum(np.linalg.norm(r.hit_xyz, axis=-1), 1.0)
-        err = np.where(m, np.linalg.norm(hx[sl] - r.hit_xyz, axis=-1) / scale, 0.0)
-        max_err = max(max_err, float(err.max()))
-        _progress(f"full form: oracle check scene {i}: {r.status.size} rays, {mism} mismatches so far")
+    check = oracle_check({int(i): scenes[int(i)] for i in check_ids}, o, d, rb, outs, mb, K, "full form")
     hits = int(ctr[L.CTR["seed_learned"]] + ctr[L.CTR["seed_fallback"]])
+    learned_value, learned_one = bounces / (te - ts), one_in_flight["value"]
+    tri = full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, learned_value, learned_one) \
+        if args.full_triangle else None
     return {
         "metric": METRIC, "value": bounces / (te - ts), "unit": "bounces/s", "n_gpus": 1, "steps": steps,
         "warmup": max(1, args.warmup), "ms_per_step": (te - ts) * 1e3 / steps, "higher_is_better": True,
@@ -494,14 +542,62 @@ def full_form_leg(args):
         "seed_fallback_per_step": int(ctr[L.CTR["seed_fallback"]]) // steps,
         "fallback_rate_per_scene": {"scenes": int(per.size), "min": float(per.min()), "median": float(np.median(per)),
                                     "mean": float(per.mean()), "max": float(per.max())} if per.size else None,
-        "oracle_check": {"scenes": 6, "rays": rays_checked, "rays_with_mismatch": mism, "max_rel_hit_error": max_err,
-                         "oracle": "CPU FP64, triangle seed, converged Newton"},
+        "oracle_check": check, "triangle_seed": tri,
         "rays_by_status": status_counts(outs["status"]), "errors_by_kind": error_counts(outs["err_kind"]),
         "bounces_per_step": bounces // steps, "train": {"pairs": rep["n_pairs"], "steps": rep["steps"],
                                                          "test_loss": rep["test_loss"]},
         "scene_generation_s": t1 - t0, "scene_build_s": t2 - t1, "train_s": t3 - t2,
         "counters_per_step": {k: int(ctr[v] // steps) for k, v in L.CTR.items()},
     }
+
+
+def full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, learned_value, learned_one):
+    """The full form's workload with the triangle seed on the same handle, rays and bounce limit, timed the same
+    way (autotuned shape, hipGraph replays one at a time, then traces in flight): the learned path's ratio on
+    this workload.  Its records must equal the learned run's (the acceptance rule keeps every learned result one
+    the triangle path gives)."""
+    import numpy as np
+    import torch
+    from billardballraytracing3d_amd import _lib as L
+
+    keep = {k: outs[k].clone() for k in ("status", "err_kind", "bounces", "hit_patch", "hit_xyz")}
+    best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st)
+
+    def step():
+        return gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, record_k=K, ray_begin=rb, stream=st,
+                               flags=L.FLAG_GRAPH, **best)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    ctr = np.zeros(L.NCOUNTERS, np.int64)
+    ts = time.perf_counter()
+    for _ in range(steps):
+        c, _ = step()
+        ctr += c
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    one = int(ctr[L.CTR["bounces"]]) / (te - ts)
+    value, ms = one, (te - ts) * 1e3 / steps
+    if args.full_in_flight > 1:
+        dt, cd, _ = run_in_flight(gb, to, td, n, mb, outs, steps, max(1, args.warmup), args.full_in_flight,
+                                  ctr // steps, dict(record_k=K, ray_begin=rb, flags=group_flag(args), **best))
+        value, ms = int(cd[:, L.CTR["bounces"]].sum()) / dt, dt * 1e3 / steps
+    slot = torch.arange(K, device=outs["bounces"].device)[None, :]
+    valid = slot < torch.clamp(keep["bounces"], max=K)[:, None].to(slot.dtype)
+    same = (outs["status"] == keep["status"]) & (outs["err_kind"] == keep["err_kind"]) & \
+        (outs["bounces"] == keep["bounces"]) & torch.all((outs["hit_patch"] == keep["hit_patch"]) | ~valid, dim=1)
+    diff = (outs["hit_xyz"] - keep["hit_xyz"]).abs().amax(dim=2)
+    max_diff = float(torch.where(valid, diff, torch.zeros_like(diff)).max().item())
+    calls = int(ctr[L.CTR["newton_calls"]])
+    _progress(f"full form, triangle seed: {value:.4g} bounces/s ({args.full_in_flight} in flight), "
+              f"{one:.4g} one at a time")
+    return {"value": value, "ms_per_step": ms, "one_in_flight": {"value": one, "ms_per_step": (te - ts) * 1e3 / steps},
+            "autotuned": best, "learned_vs_triangle": learned_value / value,
+            "learned_vs_triangle_one_in_flight": learned_one / one,
+            "mean_newton_iters": int(ctr[L.CTR["newton_iters"]]) / calls if calls else None,
+            "records_vs_learned": {"rays": n, "rays_with_mismatch": int((~same).sum().item()),
+                                   "max_abs_hit_diff": max_diff, "bounces_compared": K}}
 
 
 MLP_FLOPS_PER_WAVE_EVAL = 160 * 16 * 16 * 16 * 2   # 160 bf16 16x16x16 matrix products per 64-lane evaluation
@@ -581,15 +677,16 @@ def learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value_tri
     }
 
 
-def committed_traffic(args, n, mb, K):
-    """HBM bytes per k_persist launch measured by rocprofv3 for the default workload, or None."""
+def committed_profile_value(args, n, mb, K, key):
+    """A per-launch figure of k_persist measured by rocprofv3 for the default workload (profiles/k_persist_sinai_1M.md:
+    traffic_bytes_per_launch, fp64_valu_lane_flops_per_launch), or None for any other workload."""
     if (n, mb, K, args.mode, args.seed, args.no_cull, args.wavefront) != (1 << 20, 100, 100, "converged", 0, False, False):
         return None
     path = os.path.join(ROOT, "profiles", "k_persist_sinai_1M.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
