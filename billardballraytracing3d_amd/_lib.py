@@ -36,7 +36,8 @@ EXPORTS = ("bbrt_scene_create", "bbrt_scene_destroy", "bbrt_scene_info", "bbrt_s
            "bbrt_mlp_forward", "bbrt_mlp_loss_grad", "bbrt_mlp_train", "bbrt_mlp_sample_pairs", "bbrt_mlp_features",
            "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab", "bbrt_scene_create_batch", "bbrt_scene_batch_info",
            "bbrt_trace_batch", "bbrt_trace_batch_device", "bbrt_autotune", "bbrt_scene_create_ex",
-           "bbrt_trace_device_async", "bbrt_trace_device_noalloc")
+           "bbrt_trace_device_async", "bbrt_trace_device_noalloc", "bbrt_moller_trumbore", "bbrt_triangles_create",
+           "bbrt_triangles_destroy", "bbrt_triangles_intersect", "bbrt_scene_intersect")
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _d = C.POINTER(C.c_double)
@@ -132,6 +133,13 @@ def lib():
                                               C.POINTER(Outputs), C.c_void_p, C.c_void_p]
         L.bbrt_trace_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Params),
                                                 C.POINTER(Outputs), C.c_void_p, C.c_void_p]
+        L.bbrt_moller_trumbore.argtypes = [_d, _d, _d, C.c_int64, C.c_double, _i8, _d, _d, _d]
+        L.bbrt_triangles_create.argtypes = [_d, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
+        L.bbrt_triangles_destroy.argtypes = [C.c_void_p]
+        L.bbrt_triangles_destroy.restype = None
+        L.bbrt_triangles_intersect.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.c_int32, _i32, _d, _d, _d]
+        L.bbrt_scene_intersect.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.c_int32, C.c_int32, _i32, _d, _d, _d, _i8,
+                                           _i32]
         L.bbrt_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.c_int64, C.POINTER(Params),
                                     C.POINTER(Outputs), C.POINTER(Params), C.POINTER(TuneEntry), C.c_int32, _i32,
                                     C.c_void_p]

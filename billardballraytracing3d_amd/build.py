@@ -35,7 +35,7 @@ def build(verbose: bool = False) -> str:
     o_host = os.path.join(BUILD, "scene_build.o")
     _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", o_host], verbose)
     objs = []
-    for name in ("bbrt_hip", "bbrt_mlp"):
+    for name in ("bbrt_hip", "bbrt_mlp", "bbrt_objects"):
         o = os.path.join(BUILD, name + ".o")
         _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, name + ".hip"), "-o", o], verbose)
         objs.append(o)

@@ -59,6 +59,56 @@ def build_bvh(box, cent, leaf_threshold: int = 4):
     return dict(box=nb[:nn], right=r[:nn], start=s[:nn], count=c[:nn], axis=a[:nn], leaf=lo)
 
 
+def moller_trumbore(tri, origins, dirs, eps: float = 1e-9):
+    """moller_trumbore (MollerTrumbore.hpp:32-64) on (ray, triangle) pairs on the GPU: tri [n][9] (v0, v1, v2),
+    origins / dirs [n][3] -> (hit [n] bool, t [n], point [n][3], normal [n][3])."""
+    tri = np.ascontiguousarray(tri, dtype=np.float64).reshape(-1, 9)
+    o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    n = tri.shape[0]
+    if o.shape[0] != n or d.shape[0] != n:
+        raise ValueError("tri, origins and dirs must have the same length")
+    hit = np.zeros(n, np.int8); t = np.zeros(n); p = np.zeros((n, 3)); nn = np.zeros((n, 3))
+    L.check(L.lib().bbrt_moller_trumbore(L.ptr(tri, L._d), L.ptr(o, L._d), L.ptr(d, L._d), C.c_int64(n), C.c_double(eps),
+                                         L.ptr(hit, L._i8), L.ptr(t, L._d), L.ptr(p, L._d), L.ptr(nn, L._d)),
+            "bbrt_moller_trumbore")
+    return hit.astype(bool), t, p, nn
+
+
+class Triangles:
+    """A set of flat triangles with its BVH on the GPU (bbrt_triangles): BVH::intersect or brute_force_intersect
+    (BVH.hpp:75-128, BruteForce.hpp:7-15) for many rays at once."""
+
+    def __init__(self, tri, leaf_threshold: int = 4):
+        tri = np.ascontiguousarray(tri, dtype=np.float64).reshape(-1, 9)
+        h = C.c_void_p()
+        L.check(L.lib().bbrt_triangles_create(L.ptr(tri, L._d), C.c_int64(tri.shape[0]), C.c_int32(leaf_threshold),
+                                              C.byref(h)), "bbrt_triangles_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            L.lib().bbrt_triangles_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def intersect(self, origins, dirs, brute_force: bool = False):
+        """-> (object [n] (-1 miss), t [n], point [n][3], normal [n][3])."""
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = o.shape[0]
+        obj = np.zeros(n, np.int32); t = np.zeros(n); p = np.zeros((n, 3)); nn = np.zeros((n, 3))
+        L.check(L.lib().bbrt_triangles_intersect(self.h, L.ptr(o, L._d), L.ptr(d, L._d), C.c_int64(n),
+                                                 C.c_int32(int(brute_force)), L.ptr(obj, L._i32), L.ptr(t, L._d),
+                                                 L.ptr(p, L._d), L.ptr(nn, L._d)), "bbrt_triangles_intersect")
+        return obj, t, p, nn
+
+
 def _params(mode, seed, max_bounces, record_k, flags, block, waves_per_eu, newton_batch, grid_fraction=0.0):
     return L.Params(newton_mode=mode, seed_mode=seed, max_bounces=max_bounces, record_k=record_k, flags=flags,
                     block=block, waves_per_eu=waves_per_eu, grid_fraction=grid_fraction, newton_batch=newton_batch)
@@ -141,6 +191,20 @@ class GpuScene:
         L.check(L.lib().bbrt_scene_coarse(self.h, C.c_int64(k), L.ptr(xyz, L._d), L.ptr(sag, L._d), L.ptr(cn, L._d)),
                 "bbrt_scene_coarse")
         return xyz, sag, cn
+
+    def intersect(self, origins, dirs, mode: int = L.NEWTON_REF, brute_force: bool = False):
+        """BVH::intersect (or brute_force_intersect) over the patches for many rays, directions as given
+        (bbrt_scene_intersect) -> dict of patch, t, point, normal, err_kind, err_patch."""
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = o.shape[0]
+        patch = np.zeros(n, np.int32); t = np.zeros(n); p = np.zeros((n, 3)); nn = np.zeros((n, 3))
+        ek = np.zeros(n, np.int8); ep = np.zeros(n, np.int32)
+        L.check(L.lib().bbrt_scene_intersect(self.h, L.ptr(o, L._d), L.ptr(d, L._d), C.c_int64(n), C.c_int32(mode),
+                                             C.c_int32(int(brute_force)), L.ptr(patch, L._i32), L.ptr(t, L._d),
+                                             L.ptr(p, L._d), L.ptr(nn, L._d), L.ptr(ek, L._i8), L.ptr(ep, L._i32)),
+                "bbrt_scene_intersect")
+        return dict(patch=patch, t=t, point=p, normal=nn, err_kind=ek, err_patch=ep)
 
     def trace(self, origins, dirs, max_bounces: int, mode: int = L.NEWTON_CONVERGED,
               record_k: Optional[int] = None, flags: int = 0, hit_count: bool = False, block: int = 0,

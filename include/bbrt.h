@@ -280,6 +280,45 @@ int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int
                        int32_t* leaf_objects);
 
 /* ------------------------------------------------------------------------
+ * Object-level intersection, for callers that intersect objects directly
+ * instead of tracing rays through a driver (the reference's own test
+ * tests/Triangle_Intersect.cpp does).  Host arrays in and out; the work runs on
+ * the GPU with the bounce loop's arithmetic.
+ * ------------------------------------------------------------------------ */
+/* Moller-Trumbore on n (ray, triangle) pairs: moller_trumbore(ray, v0, v1, v2,
+ * eps) (include/Utils/MollerTrumbore.hpp:32-64), as Triangle::intersect calls
+ * it (include/ScatObjects/Triangle.hpp:29-31).  tri [n][9] = v0, v1, v2;
+ * origins, dirs [n][3].  Out: hit [n] (1/0) and for hits t [n], point [n][3]
+ * (o + t d) and the unit face normal [n][3]. */
+int bbrt_moller_trumbore(const double* tri, const double* origins, const double* dirs, int64_t n, double eps,
+                         int8_t* hit, double* t, double* point, double* normal);
+
+/* A set of m flat triangles on the device with its BVH, built as
+ * BVH::BVH(objects, leaf_threshold) builds it (include/BVH.hpp:67-73,
+ * :136-189) from the triangles' boxes and centroids (Triangle.hpp:11-23).
+ * tri [m][9]. */
+typedef struct bbrt_triangles_s* bbrt_triangles;
+int bbrt_triangles_create(const double* tri, int64_t m, int32_t leaf_threshold, bbrt_triangles* out);
+void bbrt_triangles_destroy(bbrt_triangles triangles);
+/* BVH::intersect (BVH.hpp:75-128) over the set for n rays, or, with
+ * brute_force, brute_force_intersect (include/BruteForce.hpp:7-15) over the
+ * triangles in order.  Out: object [n] (the triangle index, -1 for a miss) and
+ * for hits t, point [n][3], normal [n][3]. */
+int bbrt_triangles_intersect(bbrt_triangles triangles, const double* origins, const double* dirs, int64_t n,
+                             int32_t brute_force, int32_t* object, double* t, double* point, double* normal);
+
+/* BVH::intersect (BVH.hpp:75-128) over a one-scene handle's patches, with
+ * SurfacePatch::intersect (include/ScatObjects/SurfacePatch.hpp:253-303) at the
+ * leaves, for n rays whose directions are used as given; or, with brute_force,
+ * brute_force_intersect (BruteForce.hpp:7-15) over the patches in index order.
+ * Out per ray: patch (-1: no hit or an error), t, point [3], normal [3] of the
+ * closest hit, and err_kind (BBRT_ERR_*) / err_patch of the intersection that
+ * threw, which ends the ray's query as the exception ends BVH::intersect. */
+int bbrt_scene_intersect(bbrt_scene scene, const double* origins, const double* dirs, int64_t n, int32_t newton_mode,
+                         int32_t brute_force, int32_t* patch, double* t, double* point, double* normal,
+                         int8_t* err_kind, int32_t* err_patch);
+
+/* ------------------------------------------------------------------------
  * Learned Newton seed.  No counterpart in the reference, whose Newton seed is
  * always the centroid of the closest stage-1 triangle
  * (SurfacePatch.hpp:272-303, :499-510).  A 51 -> 64 -> 64 -> 64 -> 3 GELU

@@ -5,13 +5,19 @@
 // reference's headers, with the per-ray trace loop replaced by one batched
 // GPU call:
 //   ScatObject, Vec3, Ray, AABB, Hit         include/ScatObject.hpp:10-89
-//   SurfacePatch (file format, data,
-//     sample_grid, max_sagitta)               include/ScatObjects/SurfacePatch.hpp:52-371
+//   moller_trumbore                           include/Utils/MollerTrumbore.hpp:32-64
+//   SurfacePatch (file format, data, intersect,
+//     coarse_grid, sample_grid, max_sagitta)  include/ScatObjects/SurfacePatch.hpp:52-371
+//   Triangle                                  include/ScatObjects/Triangle.hpp:9-32
+//   BVHNode, BVH                              include/BVH.hpp:56-189
+//   brute_force_intersect                     include/BruteForce.hpp:7-15
 //   RayTracingDriver                          include/RayTracingDriver.hpp:65-425
 //   PlaneSourceDriver                         include/PlaneSourceDriver.hpp:55-244
 //   BounceMapDriver                           include/BounceMapDriver.hpp:61-520
-// There is no CPU intersection path: SurfacePatch::intersect throws, every
-// trace runs the HIP kernels.
+// There is no CPU intersection path: every trace and every intersect() call
+// runs the HIP kernels (moller_trumbore, Triangle::intersect, BVH::intersect
+// and SurfacePatch::intersect through bbrt_moller_trumbore,
+// bbrt_triangles_intersect and bbrt_scene_intersect).
 #pragma once
 
 #include <algorithm>
@@ -27,6 +33,7 @@
 #include <iostream>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -144,6 +151,26 @@ inline void interpolate_2d(const std::vector<double>& xn, const std::vector<doub
             out[(size_t)i * Q + j] = acc * (sx[i] * sy[j]);
         }
 }
+// The messages of the intersection failures (SurfacePatch.hpp:618-646,
+// RayTracingDriver.hpp:176-226), by error kind; zone is the patch's zoneID.
+inline std::string patch_error_text(int kind, const std::string& zone) {
+    switch (kind) {
+        case BBRT_ERR_SINGULAR:
+            return "SurfacePatch::intersect: singular Jacobian during Newton refinement (zoneID " + zone +
+                   "), likely a grazing ray";
+        case BBRT_ERR_NOCONV:
+            return "SurfacePatch::intersect: Newton refinement failed to converge in 20 iterations (zoneID " + zone + ")";
+        case BBRT_ERR_DOMAIN:
+            return "SurfacePatch::intersect: Newton refinement converged outside the patch domain (zoneID " + zone + ")";
+        case BBRT_ERR_BACKFACE:
+            return "RayTracingDriver::trace_ray: hit back face of patch (dot(dir, normal) > 0) — ray likely "
+                   "tunneled through a triangulation gap";
+        case BBRT_ERR_ZERO_DIR: return "RayTracingDriver::trace_ray: zero-length direction";
+        default: return "RayTracingDriver::trace_ray: BVH traversal stack overflow";
+    }
+}
+inline std::string patch_error_text(int kind, long long zone) { return patch_error_text(kind, std::to_string(zone)); }
+
 }  // namespace detail
 
 // --- SurfacePatch (SurfacePatch.hpp:52-371) --------------------------------
@@ -167,19 +194,79 @@ public:
         Vec3 tangent_u(int i, int j) const { const int k = i * n + j; return { dxdu[k], dydu[k], dzdu[k] }; }
     };
 
-    explicit SurfacePatch(const std::string& path) : data_(load_patch_file(path)) { compute_centroid_and_box(); }
+    // SurfacePatch.hpp:95-115
+    struct CoarseGrid {
+        int n = 0;
+        std::vector<double> u, v;          // n each: -1 + 2 i / (n - 1)
+        std::vector<double> x, y, z;       // n * n, idx = i * n + j
+        std::vector<double> cell_sagitta;  // (n - 1)^2
+        std::vector<Vec3> cell_normal;     // (n - 1)^2
+        Vec3 point(int i, int j) const { const int idx = i * n + j; return { x[idx], y[idx], z[idx] }; }
+        int cell_index(int i, int j) const { return i * (n - 1) + j; }
+    };
+    // A patch a driver constructs: its scene build fills in the inflated box and
+    // the sagitta (set_prepared), so the patch is not prepared on its own.
+    struct Deferred {};
+
+    // SurfacePatch.hpp:125-131: load, then the coarse grid's cell padding inflates
+    // the box and gives max_sagitta (computed by the library's scene preparation).
+    explicit SurfacePatch(const std::string& path) : SurfacePatch(path, Deferred{}) { prepare(); }
     // From values already parsed (the binary patch cache).
-    explicit SurfacePatch(Data d) : data_(std::move(d)) { compute_centroid_and_box(); }
+    explicit SurfacePatch(Data d) : SurfacePatch(std::move(d), Deferred{}) { prepare(); }
     SurfacePatch(const std::string& directory, const std::string& file_prefix, long long index)
         : SurfacePatch(directory + file_prefix + std::to_string(index + 1) + ".txt") {}
+    SurfacePatch(const std::string& path, Deferred) : data_(load_patch_file(path)) { compute_centroid_and_box(); }
+    SurfacePatch(Data d, Deferred) : data_(std::move(d)) { compute_centroid_and_box(); }
 
     const Data& data() const { return data_; }
     double max_sagitta() const { return max_sagitta_; }
     double self_intersect_padding() const override { return max_sagitta_; }
 
-    // The intersection runs on the GPU inside RayTracingDriver; there is no host path.
-    std::optional<Hit> intersect(const Ray&) const override {
-        throw std::logic_error("SurfacePatch::intersect: patches are intersected on the GPU by RayTracingDriver");
+    // Newton stopping rule of intersect(): BBRT_NEWTON_REF is the reference's.
+    int newton_mode = BBRT_NEWTON_REF;
+
+    // SurfacePatch.hpp:253-303 (stage 1 on the padded coarse triangles, Newton from
+    // the closest one's centroid), on the GPU: a one-patch scene of this patch,
+    // built on the first call.  Newton failures throw std::runtime_error with the
+    // reference's messages (SurfacePatch.hpp:618-646).
+    std::optional<Hit> intersect(const Ray& ray) const override {
+        const double o[3] = { ray.origin.x, ray.origin.y, ray.origin.z };
+        const double d[3] = { ray.dir.x, ray.dir.y, ray.dir.z };
+        int32_t patch = -1, err_patch = -1;
+        int8_t err_kind = 0;
+        double t = 0.0, p[3], nrm[3];
+        detail::check(bbrt_scene_intersect(device_scene(), o, d, 1, newton_mode, 1, &patch, &t, p, nrm, &err_kind,
+                                           &err_patch), "bbrt_scene_intersect");
+        if (err_kind) throw std::runtime_error(detail::patch_error_text(err_kind, data_.zoneID));
+        if (patch < 0) return std::nullopt;
+        Hit h;
+        h.t = t; h.point = { p[0], p[1], p[2] }; h.normal = { nrm[0], nrm[1], nrm[2] };
+        return h;
+    }
+
+    // SurfacePatch.hpp:142: the 100 x 100 coarse grid and its cell padding, as the
+    // library's scene preparation computes them (computed on the first call).
+    const CoarseGrid& coarse_grid() const {
+        std::call_once(coarse_->once, [this] {
+            const int n = 100;
+            std::vector<double> xyz((size_t)3 * n * n), cell((size_t)4 * (n - 1) * (n - 1));
+            OnePatchSet one(data_);
+            detail::check(bbrt_prepare_patches(&one.set, nullptr, nullptr, nullptr, xyz.data(), cell.data()),
+                          "bbrt_prepare_patches");
+            CoarseGrid& g = coarse_->grid;
+            g.n = n;
+            g.u.resize(n); g.v.resize(n);
+            for (int i = 0; i < n; ++i) g.u[i] = g.v[i] = -1.0 + 2.0 * static_cast<double>(i) / (n - 1);
+            g.x.resize((size_t)n * n); g.y.resize((size_t)n * n); g.z.resize((size_t)n * n);
+            for (size_t k = 0; k < (size_t)n * n; ++k) { g.x[k] = xyz[3 * k]; g.y[k] = xyz[3 * k + 1]; g.z[k] = xyz[3 * k + 2]; }
+            const size_t nc = (size_t)(n - 1) * (n - 1);
+            g.cell_sagitta.resize(nc); g.cell_normal.resize(nc);
+            for (size_t k = 0; k < nc; ++k) {
+                g.cell_sagitta[k] = cell[4 * k];
+                g.cell_normal[k] = { cell[4 * k + 1], cell[4 * k + 2], cell[4 * k + 3] };
+            }
+        });
+        return coarse_->grid;
     }
 
     // SurfacePatch.hpp:199-242
@@ -249,6 +336,50 @@ public:
 private:
     Data data_;
     double max_sagitta_ = 0.0;
+    struct DeviceScene {
+        std::once_flag once;
+        bbrt_scene scene = nullptr;
+        ~DeviceScene() { if (scene) bbrt_scene_destroy(scene); }
+    };
+    struct CoarseCache {
+        std::once_flag once;
+        CoarseGrid grid;
+    };
+    // A bbrt_patch_set of this one patch (SurfacePatch.hpp:74-80 field order).
+    struct OnePatchSet {
+        int32_t imax, jmax;
+        int64_t uo[2], vo[2], fo[2];
+        std::vector<double> fields;
+        bbrt_patch_set set;
+        explicit OnePatchSet(const Data& d) : imax(d.imax), jmax(d.jmax) {
+            if (d.kmax != 1) throw std::runtime_error("SurfacePatch: only surface patches (kmax == 1) are supported");
+            const size_t nn = (size_t)d.imax * d.jmax;
+            uo[0] = vo[0] = fo[0] = 0; uo[1] = d.imax; vo[1] = d.jmax; fo[1] = (int64_t)nn;
+            const std::vector<double>* src[12] = { &d.x, &d.y, &d.z, &d.dxdu, &d.dydu, &d.dzdu,
+                                                   &d.dxdv, &d.dydv, &d.dzdv, &d.nuX, &d.nuY, &d.nuZ };
+            fields.resize(12 * nn);
+            for (int f = 0; f < 12; ++f) std::copy(src[f]->begin(), src[f]->begin() + nn, fields.begin() + f * nn);
+            set = bbrt_patch_set{ 1, &imax, &jmax, uo, vo, fo, d.uNodes.data(), d.vNodes.data(), fields.data() };
+        }
+    };
+    std::shared_ptr<DeviceScene> dev_ = std::make_shared<DeviceScene>();
+    std::shared_ptr<CoarseCache> coarse_ = std::make_shared<CoarseCache>();
+
+    bbrt_scene device_scene() const {
+        std::call_once(dev_->once, [this] {
+            OnePatchSet one(data_);
+            detail::check(bbrt_scene_create(&one.set, &dev_->scene), "bbrt_scene_create");
+        });
+        return dev_->scene;
+    }
+
+    // The inflated box and max sagitta of this patch alone (SurfacePatch.hpp:446-492).
+    void prepare() {
+        OnePatchSet one(data_);
+        double box[6], cent[3], ms = 0.0;
+        detail::check(bbrt_prepare_patches(&one.set, box, cent, &ms, nullptr, nullptr), "bbrt_prepare_patches");
+        set_prepared(box, ms);
+    }
 
     // SurfacePatch.hpp:380-407 (the box is replaced by the inflated one once the scene is built)
     void compute_centroid_and_box() {
@@ -267,6 +398,198 @@ private:
         const double inv_n = 1.0 / static_cast<double>(n);
         centroid_ = { sx * inv_n, sy * inv_n, sz * inv_n };
         box_ = box;
+    }
+};
+
+// --- moller_trumbore (Utils/MollerTrumbore.hpp:32-64), on the GPU -------------
+inline std::optional<Hit> moller_trumbore(const Ray& ray, const Vec3& v0, const Vec3& v1, const Vec3& v2,
+                                          double eps = 1e-9) {
+    const double tri[9] = { v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z };
+    const double o[3] = { ray.origin.x, ray.origin.y, ray.origin.z }, d[3] = { ray.dir.x, ray.dir.y, ray.dir.z };
+    int8_t hit = 0;
+    double t = 0.0, p[3], n[3];
+    detail::check(bbrt_moller_trumbore(tri, o, d, 1, eps, &hit, &t, p, n), "bbrt_moller_trumbore");
+    if (!hit) return std::nullopt;
+    Hit h;
+    h.t = t; h.point = { p[0], p[1], p[2] }; h.normal = { n[0], n[1], n[2] };
+    return h;
+}
+
+// --- Triangle (ScatObjects/Triangle.hpp:9-32) -----------------------------------
+class Triangle : public ScatObject {
+public:
+    Triangle(Vec3 v0, Vec3 v1, Vec3 v2) : v0_(v0), v1_(v1), v2_(v2) {
+        box_.min = { std::min({ v0.x, v1.x, v2.x }), std::min({ v0.y, v1.y, v2.y }), std::min({ v0.z, v1.z, v2.z }) };
+        box_.max = { std::max({ v0.x, v1.x, v2.x }), std::max({ v0.y, v1.y, v2.y }), std::max({ v0.z, v1.z, v2.z }) };
+        centroid_ = { (v0.x + v1.x + v2.x) / 3.0, (v0.y + v1.y + v2.y) / 3.0, (v0.z + v1.z + v2.z) / 3.0 };
+    }
+    std::optional<Hit> intersect(const Ray& ray) const override { return moller_trumbore(ray, v0_, v1_, v2_); }
+    const Vec3& v0() const { return v0_; }
+    const Vec3& v1() const { return v1_; }
+    const Vec3& v2() const { return v2_; }
+private:
+    Vec3 v0_, v1_, v2_;
+};
+
+// --- brute_force_intersect (BruteForce.hpp:7-15): every object's own intersect() --
+inline std::optional<Hit> brute_force_intersect(const Ray& ray, const std::vector<const ScatObject*>& objects) {
+    std::optional<Hit> best;
+    for (const ScatObject* obj : objects) {
+        auto hit = obj->intersect(ray);
+        if (hit && (!best || hit->t < best->t)) best = hit;
+    }
+    return best;
+}
+
+// --- BVH (BVH.hpp:56-189) ---------------------------------------------------------
+struct alignas(32) BVHNode {
+    AABB box;
+    int32_t right_child;   // -1 marks a leaf
+    int32_t obj_start;     // leaf: offset into leaf_objects()
+    int32_t obj_count;     // leaf: object count, 0 for internal nodes
+    uint8_t split_axis;
+};
+
+// The reference's median-split tree (built by the library exactly as BVH::build
+// builds it, libstdc++ nth_element included) over Triangle or SurfacePatch
+// objects, intersected on the GPU: the triangles go to a device triangle set
+// (bbrt_triangles_intersect), the patches to a one-scene handle built from them
+// (bbrt_scene_intersect; its tree is the same one, with the reference's default
+// leaf threshold 4).  Other ScatObject types have no device form and are refused.
+class BVH {
+public:
+    explicit BVH(std::vector<const ScatObject*> objects, int leaf_threshold = 4)
+        : objects_(std::move(objects)), leaf_threshold_(leaf_threshold) {
+        if (objects_.empty()) throw std::invalid_argument("BVH: no objects");
+        if (leaf_threshold < 1) throw std::invalid_argument("BVH: leaf_threshold must be at least 1");
+        bool all_tri = true, all_patch = true;
+        for (const ScatObject* o : objects_) {
+            all_tri = all_tri && dynamic_cast<const Triangle*>(o) != nullptr;
+            all_patch = all_patch && dynamic_cast<const SurfacePatch*>(o) != nullptr;
+        }
+        if (!all_tri && !all_patch)
+            throw std::invalid_argument("BVH: objects must be all Triangle or all SurfacePatch (the kinds the GPU "
+                                        "intersects)");
+        const size_t n = objects_.size();
+        std::vector<double> box(6 * n), cent(3 * n);
+        for (size_t k = 0; k < n; ++k) {
+            const AABB& b = objects_[k]->bounding_box();
+            const Vec3& c = objects_[k]->centroid();
+            const double bb[6] = { b.min.x, b.min.y, b.min.z, b.max.x, b.max.y, b.max.z };
+            std::copy(bb, bb + 6, box.begin() + 6 * k);
+            cent[3 * k] = c.x; cent[3 * k + 1] = c.y; cent[3 * k + 2] = c.z;
+        }
+        std::vector<double> nb(12 * n);
+        std::vector<int32_t> right(2 * n), start(2 * n), count(2 * n), leaf(n);
+        std::vector<uint8_t> axis(2 * n);
+        const int64_t m = bbrt_build_bvh(box.data(), cent.data(), (int64_t)n, leaf_threshold, nb.data(), right.data(),
+                                         start.data(), count.data(), axis.data(), leaf.data());
+        if (m < 0) throw std::runtime_error(std::string("bbrt_build_bvh: ") + bbrt_last_error());
+        nodes_.resize((size_t)m);
+        for (int64_t k = 0; k < m; ++k) {
+            BVHNode& nd = nodes_[(size_t)k];
+            nd.box.min = { nb[6 * k], nb[6 * k + 1], nb[6 * k + 2] };
+            nd.box.max = { nb[6 * k + 3], nb[6 * k + 4], nb[6 * k + 5] };
+            nd.right_child = right[k]; nd.obj_start = start[k]; nd.obj_count = count[k]; nd.split_axis = axis[k];
+        }
+        for (int32_t i : leaf) leaf_objects_.push_back(objects_[(size_t)i]);
+        if (all_tri) {
+            std::vector<double> tri(9 * n);
+            for (size_t k = 0; k < n; ++k) {
+                const auto* t = static_cast<const Triangle*>(objects_[k]);
+                const Vec3 v[3] = { t->v0(), t->v1(), t->v2() };
+                for (int q = 0; q < 3; ++q) { tri[9 * k + 3 * q] = v[q].x; tri[9 * k + 3 * q + 1] = v[q].y; tri[9 * k + 3 * q + 2] = v[q].z; }
+            }
+            detail::check(bbrt_triangles_create(tri.data(), (int64_t)n, leaf_threshold, &tris_), "bbrt_triangles_create");
+        } else {
+            if (leaf_threshold != 4)
+                throw std::invalid_argument("BVH: a tree of SurfacePatch objects uses the reference's leaf threshold 4");
+            build_patch_scene();
+        }
+    }
+    ~BVH() {
+        if (tris_) bbrt_triangles_destroy(tris_);
+        if (scene_) bbrt_scene_destroy(scene_);
+    }
+    BVH(const BVH&) = delete;
+    BVH& operator=(const BVH&) = delete;
+
+    // BVH.hpp:75-128 for one ray.  A SurfacePatch intersection failure throws, as
+    // the exception of SurfacePatch::intersect propagates out of the reference's.
+    std::optional<Hit> intersect(const Ray& ray) const { return intersect_all({ ray }).front(); }
+
+    // The same for many rays in one GPU call.  With patches, the first ray whose
+    // query fails throws.
+    std::vector<std::optional<Hit>> intersect_all(const std::vector<Ray>& rays) const {
+        const size_t n = rays.size();
+        std::vector<double> o(3 * n), d(3 * n), t(n), p(3 * n), nrm(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            o[3 * i] = rays[i].origin.x; o[3 * i + 1] = rays[i].origin.y; o[3 * i + 2] = rays[i].origin.z;
+            d[3 * i] = rays[i].dir.x; d[3 * i + 1] = rays[i].dir.y; d[3 * i + 2] = rays[i].dir.z;
+        }
+        std::vector<int32_t> obj(n, -1), err_patch(n, -1);
+        std::vector<int8_t> err(n, 0);
+        if (tris_)
+            detail::check(bbrt_triangles_intersect(tris_, o.data(), d.data(), (int64_t)n, 0, obj.data(), t.data(),
+                                                   p.data(), nrm.data()), "bbrt_triangles_intersect");
+        else
+            detail::check(bbrt_scene_intersect(scene_, o.data(), d.data(), (int64_t)n, newton_mode, 0, obj.data(),
+                                               t.data(), p.data(), nrm.data(), err.data(), err_patch.data()),
+                          "bbrt_scene_intersect");
+        std::vector<std::optional<Hit>> out(n);
+        for (size_t i = 0; i < n; ++i) {
+            if (err[i]) {
+                const auto* sp = static_cast<const SurfacePatch*>(objects_[(size_t)err_patch[i]]);
+                throw std::runtime_error(detail::patch_error_text(err[i], sp->data().zoneID));
+            }
+            if (obj[i] < 0) continue;
+            Hit h;
+            h.t = t[i]; h.point = { p[3 * i], p[3 * i + 1], p[3 * i + 2] };
+            h.normal = { nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2] };
+            h.object = objects_[(size_t)obj[i]];
+            out[i] = h;
+        }
+        return out;
+    }
+
+    // Newton stopping rule for SurfacePatch objects (BBRT_NEWTON_REF is the reference's).
+    int newton_mode = BBRT_NEWTON_REF;
+    // The tree in the reference's depth-first layout (node 0 is the root).
+    const std::vector<BVHNode>& nodes() const { return nodes_; }
+    const std::vector<const ScatObject*>& leaf_objects() const { return leaf_objects_; }
+
+private:
+    std::vector<const ScatObject*> objects_;
+    int leaf_threshold_;
+    std::vector<BVHNode> nodes_;
+    std::vector<const ScatObject*> leaf_objects_;
+    bbrt_triangles tris_ = nullptr;
+    bbrt_scene scene_ = nullptr;
+
+    void build_patch_scene() {
+        const size_t n = objects_.size();
+        std::vector<int32_t> imax(n), jmax(n);
+        std::vector<int64_t> uo(n + 1, 0), vo(n + 1, 0), fo(n + 1, 0);
+        for (size_t k = 0; k < n; ++k) {
+            const auto& d = static_cast<const SurfacePatch*>(objects_[k])->data();
+            if (d.kmax != 1) throw std::runtime_error("BVH: only surface patches (kmax == 1) are supported");
+            imax[k] = d.imax; jmax[k] = d.jmax;
+            uo[k + 1] = uo[k] + d.imax; vo[k + 1] = vo[k] + d.jmax; fo[k + 1] = fo[k] + (int64_t)d.imax * d.jmax;
+        }
+        std::vector<double> un, vn, fields((size_t)12 * fo[n]);
+        for (size_t k = 0; k < n; ++k) {
+            const auto& d = static_cast<const SurfacePatch*>(objects_[k])->data();
+            un.insert(un.end(), d.uNodes.begin(), d.uNodes.end());
+            vn.insert(vn.end(), d.vNodes.begin(), d.vNodes.end());
+            const std::vector<double>* src[12] = { &d.x, &d.y, &d.z, &d.dxdu, &d.dydu, &d.dzdu,
+                                                   &d.dxdv, &d.dydv, &d.dzdv, &d.nuX, &d.nuY, &d.nuZ };
+            const size_t nn = (size_t)d.imax * d.jmax;
+            for (int f = 0; f < 12; ++f)
+                std::copy(src[f]->begin(), src[f]->begin() + nn, fields.begin() + 12 * fo[k] + f * nn);
+        }
+        bbrt_patch_set ps{ (int64_t)n, imax.data(), jmax.data(), uo.data(), vo.data(), fo.data(),
+                           un.data(), vn.data(), fields.data() };
+        detail::check(bbrt_scene_create(&ps, &scene_), "bbrt_scene_create");
     }
 };
 
@@ -411,20 +734,20 @@ inline std::vector<std::unique_ptr<SurfacePatch>> load_patch_list(const std::vec
     bool all_there = true;
     for (const auto& f : files) all_there = all_there && std::filesystem::is_regular_file(f);
     if (patch_cache_dir().empty() || !all_there) {   // (a missing file fails in the parser, as without a cache)
-        for (const auto& f : files) patches.push_back(std::make_unique<SurfacePatch>(f));
+        for (const auto& f : files) patches.push_back(std::make_unique<SurfacePatch>(f, SurfacePatch::Deferred{}));
         return patches;
     }
     const auto keys = patch_file_keys(files);
     const std::string cache = patch_cache_file(keys);
     std::vector<SurfacePatch::Data> data;
     if (read_patch_cache(cache, keys, data)) {
-        for (auto& d : data) patches.push_back(std::make_unique<SurfacePatch>(std::move(d)));
+        for (auto& d : data) patches.push_back(std::make_unique<SurfacePatch>(std::move(d), SurfacePatch::Deferred{}));
         if (from_cache) *from_cache = true;
         return patches;
     }
     std::vector<const SurfacePatch::Data*> ptrs;
     for (const auto& f : files) {
-        patches.push_back(std::make_unique<SurfacePatch>(f));
+        patches.push_back(std::make_unique<SurfacePatch>(f, SurfacePatch::Deferred{}));
         ptrs.push_back(&patches.back()->data());
     }
     write_patch_cache(cache, keys, ptrs);
@@ -626,22 +949,7 @@ private:
     std::string error_text(int kind, int patch) const {
         const std::string zone = (patch >= 0 && patch < (int)patches_.size())
                                      ? std::to_string(patches_[patch]->data().zoneID) : std::string("?");
-        switch (kind) {   // the throw sites' wording (SurfacePatch.hpp:618-646, RayTracingDriver.hpp:176-226)
-            case BBRT_ERR_SINGULAR:
-                return "SurfacePatch::intersect: singular Jacobian during Newton refinement (zoneID " + zone +
-                       "), likely a grazing ray";
-            case BBRT_ERR_NOCONV:
-                return "SurfacePatch::intersect: Newton refinement failed to converge in 20 iterations (zoneID " +
-                       zone + ")";
-            case BBRT_ERR_DOMAIN:
-                return "SurfacePatch::intersect: Newton refinement converged outside the patch domain (zoneID " +
-                       zone + ")";
-            case BBRT_ERR_BACKFACE:
-                return "RayTracingDriver::trace_ray: hit back face of patch (dot(dir, normal) > 0) — ray likely "
-                       "tunneled through a triangulation gap";
-            case BBRT_ERR_ZERO_DIR: return "RayTracingDriver::trace_ray: zero-length direction";
-            default: return "RayTracingDriver::trace_ray: BVH traversal stack overflow";
-        }
+        return detail::patch_error_text(kind, zone);   // the throw sites' wording
     }
   private:
 
@@ -813,7 +1121,7 @@ public:
                                         "]");
         for (int i = first_index_1based; i <= last_index_1based; ++i) {
             const std::string path = patch_directory + file_prefix + std::to_string(i) + ".txt";
-            auto sp = std::make_unique<SurfacePatch>(path);
+            auto sp = std::make_unique<SurfacePatch>(path, SurfacePatch::Deferred{});
             // Its launch nudge uses its own max sagitta (BounceMapDriver.hpp:161): the
             // scene's copy of the same file was prepared with exactly that value.
             const long long k = driver_.patch_index_of_path(path);

@@ -798,6 +798,38 @@ int orc_stage1_seed(void* s, int64_t k, const double* o, const double* d, double
     return 1;
 }
 
+// BVH::intersect (BVH.hpp:75-128) over the scene's patches for one ray, direction as given,
+// or with brute brute_force_intersect (BruteForce.hpp:7-15) over the patches in order.
+// out = t, point[3], normal[3]; *patch the closest hit's patch (-1 none); returns 0 or
+// the error kind of the intersection that threw, with *err_patch.
+int orc_scene_intersect(void* s, const double* o, const double* d, int mode, int brute, double* out, int* patch,
+                        int* err_patch) {
+    const Scene& S = *static_cast<Scene*>(s);
+    Counters C;
+    const Vec3 O = { o[0], o[1], o[2] }, D = { d[0], d[1], d[2] };
+    PatchHit best{ 0, 0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+    int32_t best_obj = -1, err_obj = -1;
+    int err = 0;
+    if (brute) {
+        for (size_t k = 0; k < S.patches.size(); ++k) {
+            const PatchHit h = patch_intersect(S.patches[k], O, D, mode, C);
+            if (h.status < 0) { err = -h.status; err_obj = (int32_t)k; break; }
+            if (h.status == 1 && (best_obj < 0 || h.t < best.t)) { best = h; best_obj = (int32_t)k; }
+        }
+    } else {
+        const PatchHit h = bvh_intersect(S.nodes, S.leaf, O, D,
+                                         [&](int32_t k) { return patch_intersect(S.patches[k], O, D, mode, C); },
+                                         best_obj, err_obj, C);
+        if (h.status < 0) err = -h.status; else best = h;
+    }
+    if (err) best_obj = -1;
+    out[0] = best.t; out[1] = best.point.x; out[2] = best.point.y; out[3] = best.point.z;
+    out[4] = best.normal.x; out[5] = best.normal.y; out[6] = best.normal.z;
+    *patch = best_obj;
+    *err_patch = err ? err_obj : -1;
+    return err;
+}
+
 // Newton alone from a given seed (SurfacePatch::newton_refine).
 int orc_newton(void* s, int64_t k, const double* o, const double* d, double u0, double v0, double t0, int mode,
                double* out, int* iters) {

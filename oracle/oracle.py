@@ -59,6 +59,9 @@ def lib():
         L.orc_patch_intersect.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_int, _d, C.POINTER(C.c_int)]
         L.orc_patch_intersect_uv.restype = C.c_int
         L.orc_patch_intersect_uv.argtypes = [C.c_void_p, C.c_int64, _d, _d, C.c_int, _d, C.POINTER(C.c_int)]
+        L.orc_scene_intersect.restype = C.c_int
+        L.orc_scene_intersect.argtypes = [C.c_void_p, _d, _d, C.c_int, C.c_int, _d, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
         L.orc_stage1_seed.restype = C.c_int
         L.orc_stage1_seed.argtypes = [C.c_void_p, C.c_int64, _d, _d, _d, C.POINTER(C.c_int)]
         L.orc_newton.restype = C.c_int
@@ -231,6 +234,14 @@ class OracleScene:
         st = lib().orc_patch_intersect_uv(self.h, C.c_int64(k), ptr(_v(o), _d), ptr(_v(d), _d), C.c_int(mode),
                                           ptr(out, _d), C.byref(it))
         return st, out[0], out[1:4], out[4:7], it.value, out[7], out[8]
+
+    def intersect(self, o, d, mode=0, brute=False):
+        """BVH::intersect (or brute_force_intersect) over the patches, direction as given:
+        (err_kind, err_patch, patch, t, point, normal)."""
+        out = np.zeros(7); pt = C.c_int(-1); ep = C.c_int(-1)
+        err = lib().orc_scene_intersect(self.h, ptr(_v(o), _d), ptr(_v(d), _d), C.c_int(mode), C.c_int(int(brute)),
+                                        ptr(out, _d), C.byref(pt), C.byref(ep))
+        return err, ep.value, pt.value, out[0], out[1:4], out[4:7]
 
     def stage1_seed(self, k, o, d):
         """Stage 1 of SurfacePatch::intersect alone: None on a miss, else (t0, u0, v0, i, j, second)."""

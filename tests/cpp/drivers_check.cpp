@@ -6,6 +6,12 @@
 //   drivers_check plane <dir> c0x c0y c0z c1x c1y c1z dx dy dz nx ny max_bounces count_direct <out>
 //   drivers_check bmap  <dir> <prefix> <first> <last> <n> <theta_n> <phi_n> <cone_deg> <max_bounces> <capture> <out>
 //   drivers_check cache <dir> <cache_dir>   (host only: the binary patch cache round trip and its staleness check)
+//   drivers_check kat                       (the known-answer scene of the reference's Triangle test: four
+//                                            triangles, four rays, BVH with leaf threshold 1 against brute force)
+//   drivers_check objects <dir> <rays.bin> <n> <newton_mode> <out>
+//                                           (the patch files as standalone SurfacePatch objects: BVH::intersect,
+//                                            brute_force_intersect and SurfacePatch::intersect of patch 0 per ray,
+//                                            and patch 0's coarse grid)
 //
 // Every run writes <out>.zone: int64 zoneID of each patch in the driver's
 // patch order (lexicographic file order).
@@ -95,6 +101,104 @@ int main(int argc, char** argv) {
             std::cout << "cache patches=" << a.size() << " first_from_cache=" << c1 << " second_from_cache=" << c2
                       << " after_change_from_cache=" << c3 << " equal=" << equal << std::endl;
             return (!c1 && c2 && !c3 && equal && c.size() == a.size()) ? 0 : 1;
+        }
+        if (cmd == "kat" && argc == 2) {
+            // The scene of the reference's tests/Triangle_Intersect.cpp: spread-out triangles, so the tree splits.
+            std::vector<Triangle> tris;
+            tris.emplace_back(Vec3{ 0, 0, 0 }, Vec3{ 1, 0, 0 }, Vec3{ 0, 1, 0 });
+            tris.emplace_back(Vec3{ 10, 0, 0 }, Vec3{ 11, 0, 0 }, Vec3{ 10, 1, 0 });
+            tris.emplace_back(Vec3{ 0, 0, 5 }, Vec3{ 1, 0, 5 }, Vec3{ 0, 1, 5 });
+            tris.emplace_back(Vec3{ -5, 0, 0 }, Vec3{ -4, 0, 0 }, Vec3{ -5, 1, 0 });
+            std::vector<const ScatObject*> objects;
+            for (auto& t : tris) objects.push_back(&t);
+            BVH bvh(objects, 1);
+            const Ray rays[4] = { { { 0.2, 0.2, -1 }, { 0, 0, 1 } }, { { 10.2, 0.2, -1 }, { 0, 0, 1 } },
+                                  { { 100, 100, -1 }, { 0, 0, 1 } }, { { 0.2, 0.2, 10 }, { 0, 0, -1 } } };
+            const double want[4] = { 1.0, 1.0, -1.0, 5.0 };   // -1: no hit
+            int failures = 0;
+            for (int r = 0; r < 4; ++r) {
+                const auto a = bvh.intersect(rays[r]);
+                const auto b = brute_force_intersect(rays[r], objects);
+                const bool agree = a.has_value() == b.has_value() && (!a || std::fabs(a->t - b->t) < 1e-9);
+                const bool right = want[r] < 0 ? !a : (a && std::fabs(a->t - want[r]) < 1e-9);
+                std::cout << "ray " << r << ": " << (agree && right ? "PASS" : "FAIL") << " bvh t=" << (a ? a->t : -1)
+                          << " brute t=" << (b ? b->t : -1) << std::endl;
+                if (!(agree && right)) ++failures;
+                if (a && a->object != objects[r == 3 ? 2 : (r == 1 ? 1 : 0)]) ++failures;
+            }
+            std::cout << (failures ? "Some tests FAILED." : "All tests passed.") << std::endl;
+            return failures;
+        }
+        if (cmd == "objects" && argc == 7) {
+            namespace fs = std::filesystem;
+            std::vector<std::string> files;
+            for (const auto& e : fs::directory_iterator(argv[2]))
+                if (e.is_regular_file() && e.path().extension() == ".txt") files.push_back(e.path().string());
+            std::sort(files.begin(), files.end());
+            std::vector<std::unique_ptr<SurfacePatch>> patches;
+            std::vector<const ScatObject*> objects;
+            const int mode = std::stoi(argv[5]);
+            for (const auto& f : files) {
+                patches.push_back(std::make_unique<SurfacePatch>(f));
+                patches.back()->newton_mode = mode;
+                objects.push_back(patches.back().get());
+            }
+            BVH bvh(objects);
+            bvh.newton_mode = mode;
+            const size_t n = std::stoul(argv[4]);
+            std::vector<double> rays(6 * n);
+            std::ifstream(argv[3], std::ios::binary).read(reinterpret_cast<char*>(rays.data()),
+                                                         (std::streamsize)(rays.size() * sizeof(double)));
+            // per ray and query (0 BVH, 1 brute force, 2 patch 0 alone): object (-1 none, -2 error), t, point, normal
+            std::vector<int32_t> obj(3 * n, -1);
+            std::vector<double> val(3 * n * 7, 0.0);
+            std::string errs;
+            auto idx = [&](const ScatObject* o) {
+                for (size_t k = 0; k < objects.size(); ++k) if (objects[k] == o) return (int32_t)k;
+                return (int32_t)-1;
+            };
+            for (size_t i = 0; i < n; ++i) {
+                const Ray ray{ { rays[6 * i], rays[6 * i + 1], rays[6 * i + 2] },
+                               { rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5] } };
+                for (int q = 0; q < 3; ++q) {
+                    try {
+                        std::optional<Hit> h = q == 0 ? bvh.intersect(ray)
+                                               : (q == 1 ? brute_force_intersect(ray, objects) : patches[0]->intersect(ray));
+                        if (h) {
+                            obj[3 * i + q] = q == 0 ? idx(h->object) : (q == 1 ? -3 : 0);
+                            const double v[7] = { h->t, h->point.x, h->point.y, h->point.z, h->normal.x, h->normal.y,
+                                                  h->normal.z };
+                            std::copy(v, v + 7, val.begin() + 7 * (3 * i + q));
+                        }
+                    } catch (const std::runtime_error& e) {
+                        obj[3 * i + q] = -2;
+                        errs += std::to_string(i) + " " + std::to_string(q) + " " + e.what() + "\n";
+                    }
+                }
+            }
+            // brute_force_intersect returns the hit, not the object (BruteForce.hpp:7-15): mark hits by -3
+            dump(std::string(argv[6]) + ".obj", obj);
+            dump(std::string(argv[6]) + ".val", val);
+            std::ofstream(std::string(argv[6]) + ".err") << errs;
+            const auto& g = patches[0]->coarse_grid();
+            std::vector<double> cg;
+            for (size_t k = 0; k < g.x.size(); ++k) { cg.push_back(g.x[k]); cg.push_back(g.y[k]); cg.push_back(g.z[k]); }
+            for (size_t k = 0; k < g.cell_sagitta.size(); ++k) {
+                cg.push_back(g.cell_sagitta[k]);
+                cg.push_back(g.cell_normal[k].x); cg.push_back(g.cell_normal[k].y); cg.push_back(g.cell_normal[k].z);
+            }
+            dump(std::string(argv[6]) + ".coarse", cg);
+            std::vector<double> meta;
+            for (const auto& p : patches) {
+                const AABB& b = p->bounding_box();
+                const double v[7] = { b.min.x, b.min.y, b.min.z, b.max.x, b.max.y, b.max.z, p->max_sagitta() };
+                meta.insert(meta.end(), v, v + 7);
+            }
+            dump(std::string(argv[6]) + ".meta", meta);
+            std::vector<int32_t> leaf;
+            for (const auto* o : bvh.leaf_objects()) leaf.push_back(idx(o));
+            dump(std::string(argv[6]) + ".leaf", leaf);
+            return 0;
         }
         if (cmd == "trace" && argc == 8) {
             RayTracingDriver drv(argv[2]);
