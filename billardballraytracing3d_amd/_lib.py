@@ -37,7 +37,7 @@ EXPORTS = ("bbrt_scene_create", "bbrt_scene_destroy", "bbrt_scene_info", "bbrt_s
            "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab", "bbrt_scene_create_batch", "bbrt_scene_batch_info",
            "bbrt_trace_batch", "bbrt_trace_batch_device", "bbrt_autotune", "bbrt_scene_create_ex",
            "bbrt_trace_device_async", "bbrt_trace_device_noalloc", "bbrt_moller_trumbore", "bbrt_triangles_create",
-           "bbrt_triangles_destroy", "bbrt_triangles_intersect", "bbrt_scene_intersect")
+           "bbrt_triangles_destroy", "bbrt_triangles_intersect", "bbrt_scene_intersect", "bbrt_bounce_map")
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _d = C.POINTER(C.c_double)
@@ -69,6 +69,17 @@ class Outputs(C.Structure):
                 ("hit_patch", C.c_void_p), ("hit_xyz", C.c_void_p), ("newton_iters", C.c_void_p),
                 ("final_od", C.c_void_p), ("hit_count", C.c_void_p), ("ray_work", C.c_void_p),
                 ("seed_stats", C.c_void_p)]
+
+
+class BounceMapParams(C.Structure):
+    _fields_ = [("grid_n", C.c_int32), ("theta_n", C.c_int32), ("phi_n", C.c_int32), ("max_bounces", C.c_int32),
+                ("newton_mode", C.c_int32), ("flags", C.c_uint32), ("cos_theta_max", C.c_double),
+                ("cos_phi", C.POINTER(C.c_double)), ("sin_phi", C.POINTER(C.c_double)), ("eps", C.POINTER(C.c_double))]
+
+
+class BounceMapOutputs(C.Structure):
+    _fields_ = [("grid", C.c_void_p), ("max_bounces", C.c_void_p), ("error_count", C.c_void_p),
+                ("best_dir", C.c_void_p), ("best_od", C.c_void_p), ("launch_o", C.c_void_p), ("launch_d", C.c_void_p)]
 
 
 class TrainConfig(C.Structure):
@@ -140,6 +151,8 @@ def lib():
         L.bbrt_triangles_intersect.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.c_int32, _i32, _d, _d, _d]
         L.bbrt_scene_intersect.argtypes = [C.c_void_p, _d, _d, C.c_int64, C.c_int32, C.c_int32, _i32, _d, _d, _d, _i8,
                                            _i32]
+        L.bbrt_bounce_map.argtypes = [C.c_void_p, _i64, C.c_int64, C.POINTER(BounceMapParams),
+                                      C.POINTER(BounceMapOutputs), _i64]
         L.bbrt_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.c_int64, C.POINTER(Params),
                                     C.POINTER(Outputs), C.POINTER(Params), C.POINTER(TuneEntry), C.c_int32, _i32,
                                     C.c_void_p]

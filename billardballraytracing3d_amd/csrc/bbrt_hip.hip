@@ -552,6 +552,155 @@ __global__ void k_zero_i32(int32_t* p, int n) {
     if (g < n) p[g] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// BounceMapDriver::run on the device (BounceMapDriver.hpp:153-238)
+// ---------------------------------------------------------------------------
+// One axis of the first-form interpolation operator at x (PatchInterpolation.hpp:54-159, as the host's
+// interpolate_2d forms it): r[k] = w_k / (x - node_k) and scale = prod_k (x - node_k), or a one-hot row
+// with scale 1 within 1e-12 of a node.
+__device__ __forceinline__ double ff_row(const double* __restrict__ nodes, const double* __restrict__ w, int n,
+                                         double x, double* r) {
+    double prod = 1.0;
+    int hot = -1;
+    for (int k = 0; k < n; ++k) {
+        const double diff = x - nodes[k];
+        if (fabs(diff) < kOnNodeTol) { hot = k; break; }
+        r[k] = w[k] / diff;
+        prod *= diff;
+    }
+    if (hot >= 0) {
+        for (int k = 0; k < n; ++k) r[k] = 0.0;
+        r[hot] = 1.0;
+        return 1.0;
+    }
+    return prod;
+}
+
+// SurfacePatch::sample_grid (SurfacePatch.hpp:199-242) at grid point (i, j) of an n x n grid: the first-form
+// value of a field, with the two products summed over their inner index upwards from 0.0.
+__device__ __forceinline__ double ff_value(const double* rx, double sx, const double* ry, double sy,
+                                           const double* __restrict__ f, int im, int jm) {
+    double acc = 0.0;
+    for (int m = 0; m < jm; ++m) {
+        double mid = 0.0;
+        for (int k = 0; k < im; ++k) mid += rx[k] * f[(size_t)k * jm + m];
+        acc += mid * ry[m];
+    }
+    return acc * (sx * sy);
+}
+
+// One thread per (sampled patch, grid point): the grid point, its twice-normalised normal (sample_grid, then
+// run()), the tangent frame (build_tangent_frame, BounceMapDriver.hpp:499-519), the origin nudged off the
+// surface by the patch's self-intersection epsilon, and the theta_n x phi_n cone of launch directions
+// (BounceMapDriver.hpp:202-220), rays point-major, theta then phi.  cos(phi) and sin(phi) come from the host
+// (its libm), so the rays are the host computation's bit for bit.
+__global__ void k_bmap_launch(DevScene S, const int64_t* __restrict__ pids, const double* __restrict__ eps,
+                              int64_t n_sample, int gn, int theta_n, int phi_n, double cos_theta_max,
+                              const double* __restrict__ cphi, const double* __restrict__ sphi, double* grid,
+                              double* o_out, double* d_out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nn = (int64_t)gn * gn;
+    if (g >= n_sample * nn) return;
+    const int64_t ps = g / nn;
+    const int k = (int)(g - ps * nn);
+    const int i = k / gn, j = k - (k / gn) * gn;
+    const PatchMeta M = S.meta[pids[ps]];
+    const int im = M.imax, jm = M.jmax;
+    const double* un = S.nodes + M.node_off;
+    const double* uw = un + im;
+    const double* vn = uw + im;
+    const double* vw = vn + jm;
+    const double* F = S.fields + M.field_off;
+    const int nf = im * jm;
+    const double ui = -1.0 + 2.0 * static_cast<double>(i) / (gn - 1);
+    const double vj = -1.0 + 2.0 * static_cast<double>(j) / (gn - 1);
+    double rx[bbrt::kMaxNodes1D], ry[bbrt::kMaxNodes1D];
+    const double sx = ff_row(un, uw, im, ui, rx);
+    const double sy = ff_row(vn, vw, jm, vj, ry);
+    // fields: x y z (0-2), dX/du (3-5), normal (9-11)
+    const V3 p = { ff_value(rx, sx, ry, sy, F, im, jm), ff_value(rx, sx, ry, sy, F + nf, im, jm),
+                   ff_value(rx, sx, ry, sy, F + 2 * nf, im, jm) };
+    const V3 du = { ff_value(rx, sx, ry, sy, F + 3 * nf, im, jm), ff_value(rx, sx, ry, sy, F + 4 * nf, im, jm),
+                    ff_value(rx, sx, ry, sy, F + 5 * nf, im, jm) };
+    const V3 ng = unit3({ ff_value(rx, sx, ry, sy, F + 9 * nf, im, jm), ff_value(rx, sx, ry, sy, F + 10 * nf, im, jm),
+                          ff_value(rx, sx, ry, sy, F + 11 * nf, im, jm) });
+    if (grid) {
+        double* q = grid + 9 * g;
+        q[0] = p.x; q[1] = p.y; q[2] = p.z; q[3] = ng.x; q[4] = ng.y; q[5] = ng.z; q[6] = du.x; q[7] = du.y; q[8] = du.z;
+    }
+    const V3 n = unit3(ng);
+    // build_tangent_frame
+    const double du_len2 = dot3(du, du);
+    const double du_dot_n = dot3(du, n);
+    const V3 du_perp = { du.x - du_dot_n * n.x, du.y - du_dot_n * n.y, du.z - du_dot_n * n.z };
+    const double du_perp_len2 = dot3(du_perp, du_perp);
+    V3 t1;
+    if (du_perp_len2 > 1e-6 * 1e-6 * du_len2) {
+        t1 = unit3(du_perp);
+    } else {
+        const V3 a = (fabs(n.x) <= fabs(n.y) && fabs(n.x) <= fabs(n.z)) ? V3{ 1.0, 0.0, 0.0 }
+                     : (fabs(n.y) <= fabs(n.z) ? V3{ 0.0, 1.0, 0.0 } : V3{ 0.0, 0.0, 1.0 });
+        t1 = unit3(cross3(n, a));
+    }
+    const V3 t2 = cross3(n, t1);
+    const double e = eps[ps];
+    const V3 org = { p.x + n.x * e, p.y + n.y * e, p.z + n.z * e };
+    if (!o_out) return;
+    int64_t r = g * theta_n * phi_n;
+    for (int ti = 0; ti < theta_n; ++ti) {
+        const double mu = cos_theta_max + (static_cast<double>(ti) + 0.5) / static_cast<double>(theta_n) *
+                                              (1.0 - cos_theta_max);
+        const double sq = 1.0 - mu * mu;
+        const double st = sqrt(0.0 < sq ? sq : 0.0);
+        for (int pj = 0; pj < phi_n; ++pj, ++r) {
+            const double cp = cphi[pj], sp = sphi[pj];
+            o_out[3 * r] = org.x; o_out[3 * r + 1] = org.y; o_out[3 * r + 2] = org.z;
+            d_out[3 * r] = t1.x * st * cp + t2.x * st * sp + n.x * mu;
+            d_out[3 * r + 1] = t1.y * st * cp + t2.y * st * sp + n.y * mu;
+            d_out[3 * r + 2] = t1.z * st * cp + t2.z * st * sp + n.z * mu;
+        }
+    }
+}
+
+// Per grid point over its directions (BounceMapDriver.hpp:214-236): Error directions are counted and
+// skipped; the largest bounce count of the others, and the first direction that reaches it (strictly larger
+// than the best so far).  best_od: that direction's launch ray.
+__global__ void k_bmap_reduce(const int8_t* __restrict__ status, const int32_t* __restrict__ bounces,
+                              const double* __restrict__ o, const double* __restrict__ d, int64_t n_points,
+                              int per, int32_t* max_b, int32_t* err_c, int32_t* best_dir, double* best_od) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_points) return;
+    int best = 0, n_err = 0, arg = -1, arg_b = 0;
+    for (int q = 0; q < per; ++q) {
+        const int64_t r = g * per + q;
+        if (status[r] == bbrt::kError) { ++n_err; continue; }
+        const int b = bounces[r];
+        if (arg < 0 || b > arg_b) { arg = q; arg_b = b; }
+        best = best < b ? b : best;
+    }
+    if (max_b) max_b[g] = best;
+    if (err_c) err_c[g] = n_err;
+    if (best_dir) best_dir[g] = arg;
+    if (best_od) {
+        double* q = best_od + 6 * g;
+        const int64_t r = g * per + (arg < 0 ? 0 : arg);
+        for (int c = 0; c < 3; ++c) { q[c] = arg < 0 ? 0.0 : o[3 * r + c]; q[3 + c] = arg < 0 ? 0.0 : d[3 * r + c]; }
+    }
+}
+
+// Device buffers of one call, freed when it returns.
+struct CallBuffers {
+    std::vector<void*> p;
+    ~CallBuffers() { for (void* q : p) (void)hipFree(q); }
+    template <class T>
+    T* get(size_t count) {
+        void* q = nullptr;
+        check(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        p.push_back(q);
+        return static_cast<T*>(q);
+    }
+};
+
 }  // namespace
 
 // Scratch of the no-allocation entry point, bbrt_trace_device_noalloc (file scope;
@@ -1463,6 +1612,84 @@ int bbrt_trace_device_noalloc(bbrt_scene s, const double* origins, const double*
         char* base = noalloc_scratch(s->device);
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
         trace_impl(s, origins, dirs, n, p, out, nullptr, st, nullptr, false, nullptr, true, d_counters, base);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int bbrt_bounce_map(bbrt_scene s, const int64_t* patches, int64_t n_sample, const bbrt_bounce_map_params* bp,
+                    const bbrt_bounce_map_outputs* out, int64_t* counters) {
+    if (!s || !bp || (n_sample > 0 && !patches)) { g_last_error = "bbrt_bounce_map: NULL argument"; return -1; }
+    try {
+        if (s->n_scenes > 1) throw std::runtime_error("bbrt_bounce_map: one-scene handles only");
+        if (bp->grid_n < 2) throw std::runtime_error("bbrt_bounce_map: grid_n must be >= 2");
+        if (bp->theta_n < 1 || bp->phi_n < 1) throw std::runtime_error("bbrt_bounce_map: theta_n and phi_n must be >= 1");
+        if (!bp->cos_phi || !bp->sin_phi || !bp->eps) throw std::runtime_error("bbrt_bounce_map: NULL table");
+        if (n_sample < 0) throw std::runtime_error("bbrt_bounce_map: negative patch count");
+        for (int64_t q = 0; q < n_sample; ++q)
+            if (patches[q] < 0 || patches[q] >= s->n_patches)
+                throw std::runtime_error("bbrt_bounce_map: patch index out of range");
+        const int64_t nn = (int64_t)bp->grid_n * bp->grid_n;
+        const int64_t n_points = n_sample * nn;
+        const int per = bp->theta_n * bp->phi_n;
+        const int64_t n_rays = n_points * per;
+        if (n_rays > (int64_t)INT32_MAX - 1024) throw std::runtime_error("bbrt_bounce_map: too many launch rays");
+        if (n_points == 0) {
+            if (counters) std::memset(counters, 0, BBRT_NCOUNTERS * sizeof(int64_t));
+            return 0;
+        }
+        check(hipSetDevice(s->device), "hipSetDevice");
+        CallBuffers mem;
+        hipStream_t st = s->stream;
+        int64_t* d_pid = mem.get<int64_t>(n_sample);
+        double* d_eps = mem.get<double>(n_sample);
+        double* d_cp = mem.get<double>(bp->phi_n);
+        double* d_sp = mem.get<double>(bp->phi_n);
+        check(hipMemcpyAsync(d_pid, patches, n_sample * sizeof(int64_t), hipMemcpyHostToDevice, st), "H2D");
+        check(hipMemcpyAsync(d_eps, bp->eps, n_sample * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+        check(hipMemcpyAsync(d_cp, bp->cos_phi, bp->phi_n * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+        check(hipMemcpyAsync(d_sp, bp->sin_phi, bp->phi_n * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
+        double* d_grid = (out && out->grid) ? mem.get<double>(9 * (size_t)n_points) : nullptr;
+        double* d_o = mem.get<double>(3 * (size_t)n_rays);
+        double* d_d = mem.get<double>(3 * (size_t)n_rays);
+        int8_t* d_st = mem.get<int8_t>(n_rays);
+        int32_t* d_bc = mem.get<int32_t>(n_rays);
+        DevScene dev = s->dev;
+        dev.ray_scene = nullptr;
+        hipLaunchKernelGGL(k_bmap_launch, dim3((unsigned)((n_points + 255) / 256)), dim3(256), 0, st, dev, d_pid, d_eps,
+                           n_sample, bp->grid_n, bp->theta_n, bp->phi_n, bp->cos_theta_max, d_cp, d_sp, d_grid, d_o, d_d);
+        check(hipGetLastError(), "k_bmap_launch");
+        bbrt_params p{};
+        p.newton_mode = bp->newton_mode;
+        p.seed_mode = BBRT_SEED_TRIANGLE;
+        p.max_bounces = bp->max_bounces;
+        p.record_k = 0;
+        p.flags = bp->flags & ~(uint32_t)(BBRT_FLAG_GRAPH | BBRT_FLAG_TIME_KERNELS);
+        bbrt_outputs to{};
+        to.status = d_st;
+        to.bounces = d_bc;
+        trace_impl(s, d_o, d_d, n_rays, &p, &to, counters, st, nullptr, false, nullptr);
+        int32_t* d_mb = mem.get<int32_t>(n_points);
+        int32_t* d_ec = mem.get<int32_t>(n_points);
+        int32_t* d_bd = mem.get<int32_t>(n_points);
+        double* d_bod = (out && out->best_od) ? mem.get<double>(6 * (size_t)n_points) : nullptr;
+        hipLaunchKernelGGL(k_bmap_reduce, dim3((unsigned)((n_points + 255) / 256)), dim3(256), 0, st, d_st, d_bc, d_o, d_d,
+                           n_points, per, d_mb, d_ec, d_bd, d_bod);
+        check(hipGetLastError(), "k_bmap_reduce");
+        auto down = [&](void* h, const void* dsrc, size_t bytes) {
+            if (h) check(hipMemcpyAsync(h, dsrc, bytes, hipMemcpyDeviceToHost, st), "D2H");
+        };
+        if (out) {
+            down(out->grid, d_grid, 9 * (size_t)n_points * sizeof(double));
+            down(out->max_bounces, d_mb, (size_t)n_points * sizeof(int32_t));
+            down(out->error_count, d_ec, (size_t)n_points * sizeof(int32_t));
+            down(out->best_dir, d_bd, (size_t)n_points * sizeof(int32_t));
+            down(out->best_od, d_bod, 6 * (size_t)n_points * sizeof(double));
+            down(out->launch_o, d_o, 3 * (size_t)n_rays * sizeof(double));
+            down(out->launch_d, d_d, 3 * (size_t)n_rays * sizeof(double));
+        }
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
         return 0;
     } catch (const std::exception& e) {
         return fail(e);

@@ -192,6 +192,38 @@ class GpuScene:
                 "bbrt_scene_coarse")
         return xyz, sag, cn
 
+    def bounce_map(self, patches, grid_n: int, theta_n: int, phi_n: int, cone_angle_deg: float = 180.0,
+                   max_bounces: int = 50, mode: int = L.NEWTON_REF, launch_rays: bool = False):
+        """BounceMapDriver::run (BounceMapDriver.hpp:103-259) on the device for the given patch indices
+        (bbrt_bounce_map).  The launch nudge of each patch is the driver's self_intersect_epsilon,
+        max(1e-9 * diagonal, 5 * its max sagitta); cos(theta_max) and the phi table use the C library's cos/sin.
+        Returns a dict of grid [m][n*n][9], max_bounces, error_count, best_dir [m][n*n], best_od [m][n*n][6],
+        counters, and with launch_rays the launch origins / directions."""
+        import math
+        pid = np.ascontiguousarray(patches, dtype=np.int64)
+        m = pid.size
+        info = self.info()
+        _, _, ms = self.patch_info()
+        eps = np.array([ms[k] * 5.0 if not (5.0 * ms[k] < 1e-9 * info["diag"]) else 1e-9 * info["diag"] for k in pid])
+        kTwoPi = 6.283185307179586476925286766559
+        cphi = np.array([math.cos((j + 0.5) * kTwoPi / phi_n) for j in range(phi_n)])
+        sphi = np.array([math.sin((j + 0.5) * kTwoPi / phi_n) for j in range(phi_n)])
+        bp = L.BounceMapParams(grid_n, theta_n, phi_n, max_bounces, mode, 0,
+                               math.cos(cone_angle_deg / 2.0 * kTwoPi / 360.0), L.ptr(cphi, L._d), L.ptr(sphi, L._d),
+                               L.ptr(eps, L._d))
+        npts = m * grid_n * grid_n
+        nr = npts * theta_n * phi_n
+        res = dict(grid=np.zeros((m, grid_n * grid_n, 9)), max_bounces=np.zeros((m, grid_n * grid_n), np.int32),
+                   error_count=np.zeros((m, grid_n * grid_n), np.int32), best_dir=np.zeros((m, grid_n * grid_n), np.int32),
+                   best_od=np.zeros((m, grid_n * grid_n, 6)), counters=np.zeros(L.NCOUNTERS, np.int64), eps=eps)
+        if launch_rays:
+            res["launch_o"] = np.zeros((nr, 3)); res["launch_d"] = np.zeros((nr, 3))
+        out = L.BounceMapOutputs(*[L.vptr(res.get(k)) for k in ("grid", "max_bounces", "error_count", "best_dir",
+                                                                 "best_od", "launch_o", "launch_d")])
+        L.check(L.lib().bbrt_bounce_map(self.h, L.ptr(pid, L._i64), C.c_int64(m), C.byref(bp), C.byref(out),
+                                        L.ptr(res["counters"], L._i64)), "bbrt_bounce_map")
+        return res
+
     def intersect(self, origins, dirs, mode: int = L.NEWTON_REF, brute_force: bool = False):
         """BVH::intersect (or brute_force_intersect) over the patches for many rays, directions as given
         (bbrt_scene_intersect) -> dict of patch, t, point, normal, err_kind, err_patch."""

@@ -279,6 +279,48 @@ int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int
                        double* node_box, int32_t* right, int32_t* start, int32_t* count, uint8_t* axis,
                        int32_t* leaf_objects);
 
+/* BounceMapDriver::run (include/BounceMapDriver.hpp:103-259) on the device for
+ * n_sample patches of a one-scene handle (patches [n_sample], host array of
+ * patch indices): per patch an n x n grid of surface points (sample_grid,
+ * SurfacePatch.hpp:199-242, u, v = -1 + 2 i / (n - 1)), each launched from its
+ * point nudged along the normal by eps[patch] (the driver's
+ * self_intersect_epsilon of that patch, BounceMapDriver.hpp:161) into a
+ * theta_n x phi_n solid-angle-uniform cone about the normal in the tangent
+ * frame of build_tangent_frame (:202-220, :499-519); rays point-major, theta
+ * then phi.  cos_theta_max and the cos/sin tables of the phi_n angles
+ * (j + 0.5) 2 pi / phi_n come from the caller's libm, so the launch rays equal
+ * a host computation's bit for bit.  All rays are traced in one persistent
+ * launch; per point the largest bounce count of the directions that did not
+ * end in an Error, the Error count, and the first direction reaching that
+ * count (:214-236) are reduced on the device. */
+typedef struct {
+    int32_t grid_n;        /* position_grid_n, >= 2 */
+    int32_t theta_n, phi_n;
+    int32_t max_bounces;
+    int32_t newton_mode;   /* BBRT_NEWTON_* */
+    uint32_t flags;        /* BBRT_FLAG_* of the trace (graph and kernel timing are ignored) */
+    double cos_theta_max;  /* cos(cone_angle_deg / 2) */
+    const double* cos_phi; /* [phi_n] */
+    const double* sin_phi; /* [phi_n] */
+    const double* eps;     /* [n_sample] launch nudge of each sampled patch */
+} bbrt_bounce_map_params;
+/* Host outputs, any may be NULL: grid [n_sample][n*n][9] (point, sample_grid's
+ * unit normal, dX/du); max_bounces, error_count [n_sample][n*n]; best_dir
+ * [n_sample][n*n] (theta * phi_n + phi, -1 when every direction errored);
+ * best_od [n_sample][n*n][6] (that direction's launch origin and direction);
+ * launch_o, launch_d [n_sample*n*n*theta_n*phi_n][3]. */
+typedef struct {
+    double* grid;
+    int32_t* max_bounces;
+    int32_t* error_count;
+    int32_t* best_dir;
+    double* best_od;
+    double* launch_o;
+    double* launch_d;
+} bbrt_bounce_map_outputs;
+int bbrt_bounce_map(bbrt_scene scene, const int64_t* patches, int64_t n_sample, const bbrt_bounce_map_params* params,
+                    const bbrt_bounce_map_outputs* out, int64_t* counters);
+
 /* ------------------------------------------------------------------------
  * Object-level intersection, for callers that intersect objects directly
  * instead of tracing rays through a driver (the reference's own test

@@ -1126,6 +1126,7 @@ public:
             // scene's copy of the same file was prepared with exactly that value.
             const long long k = driver_.patch_index_of_path(path);
             if (k < 0) throw std::runtime_error("BounceMapDriver: sampled patch is not in the scene directory: " + path);
+            sample_index_.push_back(k);
             const SurfacePatch& twin = *driver_.patches()[k];
             const AABB& b = twin.bounding_box();
             const double box[6] = { b.min.x, b.min.y, b.min.z, b.max.x, b.max.y, b.max.z };
@@ -1134,117 +1135,98 @@ public:
         }
     }
 
-    // BounceMapDriver.hpp:103-259: the launch rays of all sampled points go to
-    // the GPU in batches; per point the largest bounce count over directions
-    // that did not error and the error count are reduced on the host.  With
-    // capture_best_rays, the best direction of each point (first found on
-    // ties, :228-235) is traced once more with its full path recorded.
+    // BounceMapDriver.hpp:103-259 on the GPU (bbrt_bounce_map): the grid points, tangent frames and launch
+    // cones are generated on the device (the cos/sin of the phi angles and cos(theta_max) come from the host
+    // libm, as in the reference), all launch rays are traced in one persistent launch, and per point the
+    // largest bounce count over directions that did not error, the error count and the first direction
+    // reaching that count are reduced on the device.  With capture_best_rays, the best direction of each
+    // point (first found on ties, :228-235) is traced once more with its full path recorded.
     void run(int position_grid_n, int theta_n, int phi_n, double cone_angle_deg = 180.0, int max_bounces = 50,
              bool capture_best_rays = false) {
         if (position_grid_n < 2) throw std::invalid_argument("BounceMapDriver::run: position_grid_n must be >= 2");
         if (theta_n < 1 || phi_n < 1) throw std::invalid_argument("BounceMapDriver::run: theta_n and phi_n must both be >= 1");
         if (!(cone_angle_deg > 0.0) || cone_angle_deg > 180.0)
             throw std::invalid_argument("BounceMapDriver::run: cone_angle_deg must be in (0, 180]");
-        launches_ = generate_launches(position_grid_n, theta_n, phi_n, cone_angle_deg);
         capture_best_rays_ = capture_best_rays;
-        const size_t per_point = (size_t)theta_n * phi_n;
         const size_t n_patch = sample_patches_.size();
         const size_t pts_per_patch = (size_t)position_grid_n * position_grid_n;
-        const size_t n_rays = launches_.origins.size();
-        std::vector<int8_t> status(n_rays), kind(n_rays);
-        std::vector<int32_t> bounces(n_rays), err_patch(n_rays);
-        const size_t chunk = (size_t)1 << 23;
-        for (size_t a = 0; a < n_rays; a += chunk) {
-            const size_t m = std::min(chunk, n_rays - a);
-            std::vector<double> o(3 * m), d(3 * m);
-            for (size_t i = 0; i < m; ++i) {
-                const Vec3& oo = launches_.origins[a + i];
-                const Vec3& dd = launches_.dirs[a + i];
-                o[3 * i] = oo.x; o[3 * i + 1] = oo.y; o[3 * i + 2] = oo.z;
-                d[3 * i] = dd.x; d[3 * i + 1] = dd.y; d[3 * i + 2] = dd.z;
+        const size_t n_points = n_patch * pts_per_patch;
+        const size_t n_rays = n_points * (size_t)theta_n * phi_n;
+        const double half_angle_deg = cone_angle_deg / 2.0;
+        std::vector<double> cphi(phi_n), sphi(phi_n), eps(n_patch);
+        for (int pj = 0; pj < phi_n; ++pj) {
+            const double phi = (static_cast<double>(pj) + 0.5) * kTwoPi / static_cast<double>(phi_n);
+            cphi[pj] = std::cos(phi);
+            sphi[pj] = std::sin(phi);
+        }
+        for (size_t p = 0; p < n_patch; ++p) eps[p] = driver_.self_intersect_epsilon(sample_patches_[p].get());
+        bbrt_bounce_map_params bp{};
+        bp.grid_n = position_grid_n; bp.theta_n = theta_n; bp.phi_n = phi_n;
+        bp.max_bounces = std::max(0, max_bounces);
+        bp.newton_mode = driver_.newton_mode;
+        bp.cos_theta_max = std::cos(half_angle_deg * kTwoPi / 360.0);
+        bp.cos_phi = cphi.data(); bp.sin_phi = sphi.data(); bp.eps = eps.data();
+        std::vector<double> grid(9 * n_points), best_od(capture_best_rays ? 6 * n_points : 0);
+        std::vector<int32_t> maxb(n_points), errc(n_points), bestd(n_points);
+        std::vector<double> lo(keep_launch_rays ? 3 * n_rays : 0), ld(keep_launch_rays ? 3 * n_rays : 0);
+        bbrt_bounce_map_outputs bo{ grid.data(), maxb.data(), errc.data(), bestd.data(),
+                                    capture_best_rays ? best_od.data() : nullptr,
+                                    keep_launch_rays ? lo.data() : nullptr, keep_launch_rays ? ld.data() : nullptr };
+        detail::check(bbrt_bounce_map(driver_.scene(), sample_index_.data(), (int64_t)n_patch, &bp, &bo, nullptr),
+                      "bbrt_bounce_map");
+        launches_ = Launches{};
+        for (size_t p = 0; p < n_patch; ++p) {
+            SurfacePatch::SampledGrid g;
+            g.n = position_grid_n;
+            g.u.resize(position_grid_n); g.v.resize(position_grid_n);
+            for (int i = 0; i < position_grid_n; ++i)
+                g.u[i] = g.v[i] = -1.0 + 2.0 * static_cast<double>(i) / (position_grid_n - 1);
+            std::vector<double>* f[9] = { &g.x, &g.y, &g.z, &g.nx, &g.ny, &g.nz, &g.dxdu, &g.dydu, &g.dzdu };
+            for (int c = 0; c < 9; ++c) {
+                f[c]->resize(pts_per_patch);
+                for (size_t k = 0; k < pts_per_patch; ++k) (*f[c])[k] = grid[9 * (p * pts_per_patch + k) + c];
             }
-            std::vector<int8_t> st, kd;
-            std::vector<int32_t> bc, ep;
-            driver_.trace_raw(o, d, max_bounces, 0, st, kd, bc, ep, nullptr, nullptr, nullptr);
-            std::copy(st.begin(), st.end(), status.begin() + a);
-            std::copy(bc.begin(), bc.end(), bounces.begin() + a);
+            launches_.grids.push_back(std::move(g));
+        }
+        if (keep_launch_rays) {
+            launches_.origins.resize(n_rays); launches_.dirs.resize(n_rays);
+            for (size_t r = 0; r < n_rays; ++r) {
+                launches_.origins[r] = { lo[3 * r], lo[3 * r + 1], lo[3 * r + 2] };
+                launches_.dirs[r] = { ld[3 * r], ld[3 * r + 1], ld[3 * r + 2] };
+            }
         }
         patch_max_bounces_.assign(n_patch, std::vector<int>(pts_per_patch, 0));
         patch_error_counts_.assign(n_patch, std::vector<int>(pts_per_patch, 0));
-        std::vector<size_t> best_ray(n_patch * pts_per_patch, SIZE_MAX);
         for (size_t p = 0; p < n_patch; ++p)
             for (size_t k = 0; k < pts_per_patch; ++k) {
-                int best = 0, n_errors = 0;
-                size_t arg = SIZE_MAX;
-                for (size_t r = 0; r < per_point; ++r) {
-                    const size_t ray = (p * pts_per_patch + k) * per_point + r;
-                    if (status[ray] == BBRT_ERROR) { ++n_errors; continue; }
-                    if (arg == SIZE_MAX || bounces[ray] > bounces[arg]) arg = ray;
-                    best = std::max(best, (int)bounces[ray]);
-                }
-                patch_max_bounces_[p][k] = best;
-                patch_error_counts_[p][k] = n_errors;
-                best_ray[p * pts_per_patch + k] = arg;
+                patch_max_bounces_[p][k] = maxb[p * pts_per_patch + k];
+                patch_error_counts_[p][k] = errc[p * pts_per_patch + k];
             }
         patch_best_paths_.clear();
         if (capture_best_rays_) {
-            std::vector<Vec3> bo, bd;
+            std::vector<Vec3> bo_, bd_;
             std::vector<size_t> where;
-            for (size_t q = 0; q < best_ray.size(); ++q)
-                if (best_ray[q] != SIZE_MAX) {
-                    bo.push_back(launches_.origins[best_ray[q]]);
-                    bd.push_back(launches_.dirs[best_ray[q]]);
+            for (size_t q = 0; q < n_points; ++q)
+                if (bestd[q] >= 0) {
+                    bo_.push_back({ best_od[6 * q], best_od[6 * q + 1], best_od[6 * q + 2] });
+                    bd_.push_back({ best_od[6 * q + 3], best_od[6 * q + 4], best_od[6 * q + 5] });
                     where.push_back(q);
                 }
-            const auto paths = driver_.trace_batch(bo, bd, max_bounces);
+            const auto paths = driver_.trace_batch(bo_, bd_, max_bounces);
             patch_best_paths_.assign(n_patch, std::vector<RayTracingDriver::RayPath>(pts_per_patch));
             for (size_t i = 0; i < where.size(); ++i)
                 patch_best_paths_[where[i] / pts_per_patch][where[i] % pts_per_patch] = paths[i];
         }
     }
 
+    // Keep every launch ray of the last run() in launches() (off by default: production runs have tens of
+    // millions of them; the per-point grids are always kept).
+    bool keep_launch_rays = false;
+
     struct Launches {
         std::vector<Vec3> origins, dirs;
         std::vector<SurfacePatch::SampledGrid> grids;
     };
-
-    // BounceMapDriver.hpp:115-220: per sampled patch an n x n grid of surface
-    // points, each nudged off the surface along its normal, times a
-    // theta_n x phi_n solid-angle-uniform cone about the normal.
-    Launches generate_launches(int position_grid_n, int theta_n, int phi_n, double cone_angle_deg) const {
-        Launches L;
-        const double half_angle_deg = cone_angle_deg / 2.0;
-        const double cos_theta_max = std::cos(half_angle_deg * kTwoPi / 360.0);
-        for (size_t p = 0; p < sample_patches_.size(); ++p) {
-            const auto& patch = *sample_patches_[p];
-            const double eps = driver_.self_intersect_epsilon(sample_patches_[p].get());
-            SurfacePatch::SampledGrid grid = patch.sample_grid(position_grid_n);
-            const size_t nn = grid.x.size();
-            for (size_t k = 0; k < nn; ++k) {
-                const Vec3 point = { grid.x[k], grid.y[k], grid.z[k] };
-                const Vec3 normal = vec_normalize({ grid.nx[k], grid.ny[k], grid.nz[k] });
-                const Vec3 du = { grid.dxdu[k], grid.dydu[k], grid.dzdu[k] };
-                Vec3 t1, t2;
-                build_tangent_frame(normal, du, t1, t2);
-                const Vec3 origin = { point.x + normal.x * eps, point.y + normal.y * eps, point.z + normal.z * eps };
-                for (int ti = 0; ti < theta_n; ++ti) {
-                    const double mu = cos_theta_max + (static_cast<double>(ti) + 0.5) / static_cast<double>(theta_n) *
-                                                          (1.0 - cos_theta_max);
-                    const double sin_theta = std::sqrt(std::max(0.0, 1.0 - mu * mu));
-                    for (int pj = 0; pj < phi_n; ++pj) {
-                        const double phi = (static_cast<double>(pj) + 0.5) * kTwoPi / static_cast<double>(phi_n);
-                        const double cos_phi = std::cos(phi), sin_phi = std::sin(phi);
-                        L.origins.push_back(origin);
-                        L.dirs.push_back({ t1.x * sin_theta * cos_phi + t2.x * sin_theta * sin_phi + normal.x * mu,
-                                           t1.y * sin_theta * cos_phi + t2.y * sin_theta * sin_phi + normal.y * mu,
-                                           t1.z * sin_theta * cos_phi + t2.z * sin_theta * sin_phi + normal.z * mu });
-                    }
-                }
-            }
-            L.grids.push_back(std::move(grid));
-        }
-        return L;
-    }
 
     const std::vector<std::vector<int>>& max_bounces() const { return patch_max_bounces_; }
     const std::vector<std::vector<int>>& error_counts() const { return patch_error_counts_; }
@@ -1329,6 +1311,7 @@ public:
 private:
     RayTracingDriver driver_;
     std::vector<std::unique_ptr<SurfacePatch>> sample_patches_;
+    std::vector<int64_t> sample_index_;   // each sampled patch's index in the driver's scene
     int first_index_1based_ = 1;
     bool capture_best_rays_ = false;
     Launches launches_;

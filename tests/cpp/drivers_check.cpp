@@ -237,6 +237,7 @@ int main(int argc, char** argv) {
             drv.write_vtk(out + ".vtk");
         } else if (cmd == "bmap" && argc == 13) {
             BounceMapDriver drv(argv[2], argv[3], std::stoi(argv[4]), std::stoi(argv[5]));
+            drv.keep_launch_rays = true;   // the launch rays are compared with the oracle's
             drv.run(std::stoi(argv[6]), std::stoi(argv[7]), std::stoi(argv[8]), std::stod(argv[9]), std::stoi(argv[10]),
                     std::stoi(argv[11]) != 0);
             const std::string out = argv[12];
