@@ -204,7 +204,9 @@ class GpuScene:
         m = pid.size
         info = self.info()
         _, _, ms = self.patch_info()
-        eps = np.array([ms[k] * 5.0 if not (5.0 * ms[k] < 1e-9 * info["diag"]) else 1e-9 * info["diag"] for k in pid])
+        # (indices out of range are left to the library to refuse)
+        sag = np.array([ms[k] if 0 <= k < ms.size else 0.0 for k in pid])
+        eps = np.array([s_ * 5.0 if not (5.0 * s_ < 1e-9 * info["diag"]) else 1e-9 * info["diag"] for s_ in sag])
         kTwoPi = 6.283185307179586476925286766559
         cphi = np.array([math.cos((j + 0.5) * kTwoPi / phi_n) for j in range(phi_n)])
         sphi = np.array([math.sin((j + 0.5) * kTwoPi / phi_n) for j in range(phi_n)])
