@@ -14,7 +14,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libbbrt_hip.so")
 
-NEWTON_REF, NEWTON_CONVERGED = 0, 1
+NEWTON_REF, NEWTON_CONVERGED, NEWTON_CONVERGED_FP32 = 0, 1, 2
 SEED_TRIANGLE, SEED_LEARNED = 0, 1
 ESCAPED, TRAPPED, ERROR = 0, 1, 2
 ERR_NONE, ERR_SINGULAR, ERR_NOCONV, ERR_DOMAIN, ERR_BACKFACE, ERR_ZERO_DIR, ERR_STACK = range(7)
@@ -25,7 +25,7 @@ SCENE_COMPACT, SCENE_DEVICE_BUILD = 1, 2
 NCOUNTERS = 16
 CTR = dict(rays=0, bounces=1, bvh_node_tests=2, candidates=3, cull_box_tests=4, mt_tests=5, newton_calls=6,
            newton_iters=7, newton_flops=8, polish_fail=9, launches=10, seed_learned=11, seed_fallback=12,
-           mlp_evals=13, mlp_lanes=14)
+           mlp_evals=13, mlp_lanes=14, newton_iters_fp32=15)
 MLP_NIN = 51
 MLP_NPARAMS = 13520
 

@@ -911,15 +911,16 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     DevScene dev = s->dev;
     dev.ray_scene = d_ray_scene;
     if (p->max_bounces < 0 || p->record_k < 0) throw std::runtime_error("bbrt_trace: negative max_bounces or record_k");
-    if (p->newton_mode != BBRT_NEWTON_REF && p->newton_mode != BBRT_NEWTON_CONVERGED)
+    if (p->newton_mode != BBRT_NEWTON_REF && p->newton_mode != BBRT_NEWTON_CONVERGED &&
+        p->newton_mode != BBRT_NEWTON_CONVERGED_FP32)
         throw std::runtime_error("bbrt_trace: unknown newton_mode");
     if (p->seed_mode != BBRT_SEED_TRIANGLE && p->seed_mode != BBRT_SEED_LEARNED)
         throw std::runtime_error("bbrt_trace: unknown seed_mode");
     const bool learned = p->seed_mode == BBRT_SEED_LEARNED;
     if (learned) {
         if (!s->mlp_ready) throw std::runtime_error("bbrt_trace: the learned seed needs a network (bbrt_scene_set_mlp)");
-        if (p->newton_mode != BBRT_NEWTON_CONVERGED)
-            throw std::runtime_error("bbrt_trace: the learned seed needs BBRT_NEWTON_CONVERGED");
+        if (p->newton_mode == BBRT_NEWTON_REF)
+            throw std::runtime_error("bbrt_trace: the learned seed needs BBRT_NEWTON_CONVERGED (or its FP32 start)");
         if (p->flags & BBRT_FLAG_WAVEFRONT)
             throw std::runtime_error("bbrt_trace: the learned seed runs in the persistent schedule only");
         if (out && out->ray_work) throw std::runtime_error("bbrt_trace: ray_work is not recorded with the learned seed");

@@ -106,6 +106,7 @@ struct Counters {
     uint32_t bvh_tests, candidates, cull_tests, mt_tests, newton_calls, newton_iters, newton_flops, polish_fail,
         bounces;
     uint32_t seed_learned, seed_fallback, mlp_evals, mlp_lanes;   // learned-seed mode only
+    uint32_t newton_iters32;                                      // BBRT_NEWTON_CONVERGED_FP32: FP32 iterations
 };
 
 // MollerTrumbore.hpp:32-64 (t only; the point and normal are unused by SurfacePatch).
@@ -680,6 +681,75 @@ __device__ __forceinline__ double field(const double* lu, const double* lv, cons
     return sum;
 }
 
+// The FP32 start of BBRT_NEWTON_CONVERGED_FP32: the second-form basis, the field
+// sum and Cramer's rule of the FP64 path, evaluated in FP32 from the FP64 nodal
+// data.  No result of this phase is kept unless the FP64 iterations after it
+// confirm it (see newton()).
+template <int N>
+__device__ __forceinline__ void basis32(const double* __restrict__ nodes, const double* __restrict__ w, int n, float x,
+                                        float* b) {
+    constexpr int M = N > 0 ? N : bbrt::kMaxNodes1D;
+    const int cnt = N > 0 ? N : n;
+    int hot = -1;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+        if (i < cnt && hot < 0 && x == (float)nodes[i]) hot = i;
+    if (hot >= 0) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) b[i] = (i == hot) ? 1.0f : 0.0f;
+        return;
+    }
+    float denom = 0.0f;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        if (i < cnt) {
+            const float li = (float)w[i] / (x - (float)nodes[i]);
+            b[i] = li;
+            denom += li;
+        } else {
+            b[i] = 0.0f;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+        if (i < cnt) b[i] /= denom;
+}
+
+template <int N>
+__device__ __forceinline__ float field32(const float* lu, const float* lv, const double* __restrict__ f, int im, int jm) {
+    constexpr int M = N > 0 ? N : bbrt::kMaxNodes1D;
+    const int ci = N > 0 ? N : im, cj = N > 0 ? N : jm;
+    if constexpr (N > 0) __builtin_amdgcn_sched_barrier(0);
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        if (i < ci) {
+            float row = 0.0f;
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < cj) row += lv[j] * (float)f[i * cj + j];
+            sum += lu[i] * row;
+        }
+    }
+    return sum;
+}
+
+struct F3 { float x, y, z; };
+
+// Cramer's rule in FP32; false when the determinant is zero or not finite.
+__device__ __forceinline__ bool solve3_32(const float J[3][3], const F3& b, float x[3]) {
+    const float a11 = J[0][0], a12 = J[0][1], a13 = J[0][2];
+    const float a21 = J[1][0], a22 = J[1][1], a23 = J[1][2];
+    const float a31 = J[2][0], a32 = J[2][1], a33 = J[2][2];
+    const float det = a11 * (a22 * a33 - a23 * a32) - a12 * (a21 * a33 - a23 * a31) + a13 * (a21 * a32 - a22 * a31);
+    if (!(fabsf(det) > 0.0f) || !isfinite(det)) return false;
+    const float det0 = b.x * (a22 * a33 - a23 * a32) - a12 * (b.y * a33 - a23 * b.z) + a13 * (b.y * a32 - a22 * b.z);
+    const float det1 = a11 * (b.y * a33 - a23 * b.z) - b.x * (a21 * a33 - a23 * a31) + a13 * (a21 * b.z - b.y * a31);
+    const float det2 = a11 * (a22 * b.z - b.y * a32) - a12 * (a21 * b.z - b.y * a31) + b.x * (a21 * a32 - a22 * a31);
+    x[0] = det0 / det; x[1] = det1 / det; x[2] = det2 / det;
+    return true;
+}
+
 // SurfacePatch.hpp:559-579
 __device__ __forceinline__ bool solve3(const double J[3][3], const V3& b, double x[3]) {
     const double a11 = J[0][0], a12 = J[0][1], a13 = J[0][2];
@@ -730,6 +800,39 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
     const uint32_t F_fin = 8 * Ni + 12 * Ni * (Ni + 1) + 9;
     int iters = 0;
     double step_norm = INFINITY;
+    if (mode == BBRT_NEWTON_CONVERGED_FP32) {
+        // FP32 iterations from the seed until the reference's step criterion holds in FP32; then the FP64
+        // loop below continues from there (its first step is then small: it makes the reference's stop,
+        // determinant and domain decisions in FP64, and the polish follows).  If the FP32 phase does not get
+        // there (no convergence, a zero or non-finite determinant or step), Newton starts over in FP64 from
+        // the seed, exactly as in BBRT_NEWTON_CONVERGED.
+        float fu = (float)u, fv = (float)v, ft = (float)t;
+        const F3 fo = { (float)o.x, (float)o.y, (float)o.z }, fd = { (float)d.x, (float)d.y, (float)d.z };
+        float flu[L], flv[L];
+        bool ok32 = false;
+        int it32 = 0;
+        while (it32 < kNewtonMaxIters) {
+            basis32<N>(un, uw, im, fu, flu);
+            basis32<N>(vn, vw, jm, fv, flv);
+            const F3 X = { field32<N>(flu, flv, F, im, jm), field32<N>(flu, flv, F + nn, im, jm),
+                           field32<N>(flu, flv, F + 2 * nn, im, jm) };
+            const F3 Xu = { field32<N>(flu, flv, F + 3 * nn, im, jm), field32<N>(flu, flv, F + 4 * nn, im, jm),
+                            field32<N>(flu, flv, F + 5 * nn, im, jm) };
+            const F3 Xv = { field32<N>(flu, flv, F + 6 * nn, im, jm), field32<N>(flu, flv, F + 7 * nn, im, jm),
+                            field32<N>(flu, flv, F + 8 * nn, im, jm) };
+            const F3 Fv = { X.x - (fo.x + ft * fd.x), X.y - (fo.y + ft * fd.y), X.z - (fo.z + ft * fd.z) };
+            const float J[3][3] = { { Xu.x, Xv.x, -fd.x }, { Xu.y, Xv.y, -fd.y }, { Xu.z, Xv.z, -fd.z } };
+            float dl[3];
+            if (!solve3_32(J, Fv, dl)) break;
+            fu -= dl[0]; fv -= dl[1]; ft -= dl[2];
+            ++it32;
+            const float sn = sqrtf(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+            if (!isfinite(sn)) break;
+            if (sn < (float)kNewtonTol) { ok32 = true; break; }
+        }
+        c.newton_iters32 += it32;
+        if (ok32) { u = fu; v = fv; t = ft; }
+    }
     for (int it = 0; it < kNewtonMaxIters; ++it) {
         double delta[3];
         if (!step(delta)) {
@@ -749,7 +852,7 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
         c.newton_iters += iters; c.newton_flops += iters * I_it;
         r.status = -bbrt::kErrDomain; r.iters = iters; return r;
     }
-    if (mode == 1) {
+    if (mode != BBRT_NEWTON_REF) {
         bool ok = false;
         for (int k = 0; k < kPolishMax; ++k) {
             double delta[3];
@@ -966,14 +1069,15 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ __forceinline__ void flush_counters(const Counters& c, unsigned long long* ctr) {
-    const unsigned long long v[13] = { c.bvh_tests, c.candidates, c.cull_tests, c.mt_tests, c.newton_calls,
+    const unsigned long long v[14] = { c.bvh_tests, c.candidates, c.cull_tests, c.mt_tests, c.newton_calls,
                                        c.newton_iters, c.newton_flops, c.polish_fail, c.bounces, c.seed_learned,
-                                       c.seed_fallback, c.mlp_evals, c.mlp_lanes };  // widened
-    const int slot[13] = { bbrt::kCtrBvh, bbrt::kCtrCand, bbrt::kCtrCull, bbrt::kCtrMt, bbrt::kCtrNewtonCalls,
+                                       c.seed_fallback, c.mlp_evals, c.mlp_lanes, c.newton_iters32 };  // widened
+    const int slot[14] = { bbrt::kCtrBvh, bbrt::kCtrCand, bbrt::kCtrCull, bbrt::kCtrMt, bbrt::kCtrNewtonCalls,
                            bbrt::kCtrNewtonIters, bbrt::kCtrNewtonFlops, bbrt::kCtrPolishFail, bbrt::kCtrBounces,
-                           BBRT_CTR_SEED_LEARNED, BBRT_CTR_SEED_FALLBACK, BBRT_CTR_MLP_EVALS, BBRT_CTR_MLP_LANES };
+                           BBRT_CTR_SEED_LEARNED, BBRT_CTR_SEED_FALLBACK, BBRT_CTR_MLP_EVALS, BBRT_CTR_MLP_LANES,
+                           BBRT_CTR_NEWTON_ITERS_FP32 };
 #pragma unroll
-    for (int q = 0; q < 13; ++q) {
+    for (int q = 0; q < 14; ++q) {
         const unsigned long long s = wave_sum(v[q]);
         if (__lane_id() == 0 && s) atomicAdd(ctr + slot[q], s);
     }

@@ -51,7 +51,13 @@ typedef struct {
 } bbrt_patch_set;
 
 enum { BBRT_NEWTON_REF = 0,          /* the reference stopping rule (step < 1e-3), SurfacePatch.hpp:598-630 */
-       BBRT_NEWTON_CONVERGED = 1 };  /* same classification, then FP64 polish to step < 1e-13*max(1,|t|) */
+       BBRT_NEWTON_CONVERGED = 1,    /* same classification, then FP64 polish to step < 1e-13*max(1,|t|) */
+       BBRT_NEWTON_CONVERGED_FP32 = 2 };  /* BBRT_NEWTON_CONVERGED whose first iterations run in FP32: FP32 from
+                                           the seed until its step is < 1e-3, then the FP64 iterations (the
+                                           reference's stop, determinant and domain decisions, then the polish);
+                                           when the FP32 phase fails, FP64 from the seed.  Classifications can
+                                           differ from BBRT_NEWTON_CONVERGED where FP32 and FP64 iterations part
+                                           ways (measured, not promised) */
 enum { BBRT_SEED_TRIANGLE = 0, BBRT_SEED_LEARNED = 1 };
 /* RayPath::Status, RayTracingDriver.hpp:69 */
 enum { BBRT_ESCAPED = 0, BBRT_TRAPPED = 1, BBRT_ERROR = 2 };
@@ -135,6 +141,9 @@ enum { BBRT_CTR_RAYS = 0, BBRT_CTR_BOUNCES = 1, BBRT_CTR_BVH_NODE_TESTS = 2, BBR
        /* learned-seed mode: stage-1 hits whose learned-seed Newton was accepted, those that fell back to the
           triangle seed, wave-level network evaluations, and lanes seeded by them */
        BBRT_CTR_SEED_LEARNED = 11, BBRT_CTR_SEED_FALLBACK = 12, BBRT_CTR_MLP_EVALS = 13, BBRT_CTR_MLP_LANES = 14,
+       /* BBRT_NEWTON_CONVERGED_FP32: FP32 Newton iterations (not in NEWTON_ITERS or NEWTON_FLOPS, which
+          count the FP64 work) */
+       BBRT_CTR_NEWTON_ITERS_FP32 = 15,
        BBRT_NCOUNTERS = 16 };
 
 /* Builds a scene: per patch the barycentric weights, centroid, node box,
