@@ -569,6 +569,9 @@ def full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, le
     from billardballraytracing3d_amd import _lib as L
 
     keep = {k: outs[k].clone() for k in ("status", "err_kind", "bounces", "hit_patch", "hit_xyz")}
+    # the triangle seed writes no per-ray learned/fallback counts
+    outs = {k: v for k, v in outs.items() if k != "seed_stats"}
+    ptrs = {k: v.data_ptr() for k, v in outs.items()}
     best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=mode, record_k=K, ray_begin=rb, stream=st)
 
     def step():

@@ -215,7 +215,7 @@ enum : int { kPhTrav = 1, kPhLeaf = 2, kPhDone = 3, kPhSeed = 4 };
 // kept only under the acceptance rule (converged, polished, within
 // mlp::kSeedWindow of the triangle seed), else Newton runs again from the
 // triangle seed and its outcome stands.
-template <int W, bool WORK, bool LEARNED, bool COMPACT>
+template <int W, bool WORK, bool LEARNED, bool COMPACT, bool F32 = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_t* __restrict__ n_live_p,
           int32_t* head, Outs O, int max_bounces, int record_k, int mode, uint32_t flags, unsigned long long* ctr,
@@ -467,18 +467,18 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 if (lane == 0) c.mlp_evals++;
                 if (need) {
                     c.mlp_lanes++;
-                    h = newton_any(S, obj, o, d, ul, vl, tl, mode, c);
+                    h = newton_any<F32>(S, obj, o, d, ul, vl, tl, mode, c);
                     const bool ok = h.status == 1 && h.polish_ok && fabs(h.u - sd.u) <= mlp::kSeedWindow &&
                                     fabs(h.v - sd.v) <= mlp::kSeedWindow;
                     if (ok) {
                         c.seed_learned++;
                     } else {
                         c.seed_fallback++;
-                        h = newton_any(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
+                        h = newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
                     }
                 }
             } else {
-                if (need) h = newton_any(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
+                if (need) h = newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
             }
             if (need) { phase = kPhLeaf; have_h = true; }
         }
@@ -992,6 +992,11 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     if (graph && wavefront) throw std::runtime_error("bbrt_trace: BBRT_FLAG_GRAPH needs the persistent schedule");
     // the persistent kernel and its grid (host-side choices, fixed in a captured graph)
     decltype(&k_persist<2, false, false, false>) kern = nullptr;
+    const bool f32 = p->newton_mode == BBRT_NEWTON_CONVERGED_FP32;
+    if (f32 && (p->flags & BBRT_FLAG_WAVEFRONT))
+        throw std::runtime_error("bbrt_trace: BBRT_NEWTON_CONVERGED_FP32 runs in the persistent schedule only");
+    if (f32 && out && out->ray_work)
+        throw std::runtime_error("bbrt_trace: ray_work is not recorded with BBRT_NEWTON_CONVERGED_FP32");
     unsigned grid = 1;
     const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
     const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch;
@@ -1000,7 +1005,20 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         const bool work = O.ray_work != nullptr;
         if (wpe != 2 && wpe != 3 && wpe != 4) throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
         if (s->compact && work) throw std::runtime_error("bbrt_trace: ray_work is not recorded on a compact scene");
-        if (s->compact) {
+        if (f32) {
+            if (s->compact && learned)
+                kern = wpe == 2 ? &k_persist<2, false, true, true, true>
+                                : (wpe == 3 ? &k_persist<3, false, true, true, true> : &k_persist<4, false, true, true, true>);
+            else if (s->compact)
+                kern = wpe == 2 ? &k_persist<2, false, false, true, true>
+                                : (wpe == 3 ? &k_persist<3, false, false, true, true> : &k_persist<4, false, false, true, true>);
+            else if (learned)
+                kern = wpe == 2 ? &k_persist<2, false, true, false, true>
+                                : (wpe == 3 ? &k_persist<3, false, true, false, true> : &k_persist<4, false, true, false, true>);
+            else
+                kern = wpe == 2 ? &k_persist<2, false, false, false, true>
+                                : (wpe == 3 ? &k_persist<3, false, false, false, true> : &k_persist<4, false, false, false, true>);
+        } else if (s->compact) {
             if (learned) kern = wpe == 2 ? &k_persist<2, false, true, true> : (wpe == 3 ? &k_persist<3, false, true, true>
                                                                                         : &k_persist<4, false, true, true>);
             else kern = wpe == 2 ? &k_persist<2, false, false, true> : (wpe == 3 ? &k_persist<3, false, false, true>

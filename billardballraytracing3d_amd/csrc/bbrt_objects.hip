@@ -314,8 +314,7 @@ int bbrt_scene_intersect(bbrt_scene s, const double* origins, const double* dirs
         return -1;
     }
     if (s->n_scenes != 1) { g_last_error = "bbrt_scene_intersect: one-scene handles only"; return -1; }
-    if (newton_mode != BBRT_NEWTON_REF && newton_mode != BBRT_NEWTON_CONVERGED &&
-        newton_mode != BBRT_NEWTON_CONVERGED_FP32) {
+    if (newton_mode != BBRT_NEWTON_REF && newton_mode != BBRT_NEWTON_CONVERGED) {
         g_last_error = "bbrt_scene_intersect: unknown newton_mode";
         return -1;
     }

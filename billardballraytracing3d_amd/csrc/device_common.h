@@ -765,8 +765,10 @@ __device__ __forceinline__ bool solve3(const double J[3][3], const V3& b, double
     return true;
 }
 
-// Newton on (u, v, t), SurfacePatch.hpp:593-660; mode 1 adds the polish.
-template <int N>
+// Newton on (u, v, t), SurfacePatch.hpp:593-660; the converged modes add the polish.  F32: the kernel carries
+// the FP32 start of BBRT_NEWTON_CONVERGED_FP32 (a separate instantiation, so the FP64-only kernels keep their
+// registers), for the 3x3 and 5x5 node sets; other node counts run FP64 throughout.
+template <int N, bool F32 = false>
 __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const V3& d, double u, double v, double t,
                        int mode, Counters& c) {
     constexpr int L = N > 0 ? N : bbrt::kMaxNodes1D;
@@ -800,7 +802,7 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
     const uint32_t F_fin = 8 * Ni + 12 * Ni * (Ni + 1) + 9;
     int iters = 0;
     double step_norm = INFINITY;
-    if (mode == BBRT_NEWTON_CONVERGED_FP32) {
+    if constexpr (F32 && N > 0) if (mode == BBRT_NEWTON_CONVERGED_FP32) {
         // FP32 iterations from the seed until the reference's step criterion holds in FP32; then the FP64
         // loop below continues from there (its first step is then small: it makes the reference's stop,
         // determinant and domain decisions in FP64, and the polish follows).  If the FP32 phase does not get
@@ -1046,12 +1048,13 @@ __device__ __forceinline__ Seed seed_of(const Stage1& best) {
 }
 
 // newton_refine from (u0, v0, t0) on patch k, specialised on the node count.
+template <bool F32 = false>
 __device__ __forceinline__ PHit newton_any(const DevScene& S, int64_t k, const V3& o, const V3& d, double u0, double v0,
                                            double t0, int mode, Counters& c) {
     const PatchMeta M = S.meta[k];
-    if (M.imax == 3 && M.jmax == 3) return newton<3>(S, M, o, d, u0, v0, t0, mode, c);
-    if (M.imax == 5 && M.jmax == 5) return newton<5>(S, M, o, d, u0, v0, t0, mode, c);
-    return newton<0>(S, M, o, d, u0, v0, t0, mode, c);
+    if (M.imax == 3 && M.jmax == 3) return newton<3, F32>(S, M, o, d, u0, v0, t0, mode, c);
+    if (M.imax == 5 && M.jmax == 5) return newton<5, F32>(S, M, o, d, u0, v0, t0, mode, c);
+    return newton<0, F32>(S, M, o, d, u0, v0, t0, mode, c);
 }
 
 // SurfacePatch::intersect (SurfacePatch.hpp:253-303): stage 1, then Newton from its seed.

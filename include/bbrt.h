@@ -57,7 +57,8 @@ enum { BBRT_NEWTON_REF = 0,          /* the reference stopping rule (step < 1e-3
                                            reference's stop, determinant and domain decisions, then the polish);
                                            when the FP32 phase fails, FP64 from the seed.  Classifications can
                                            differ from BBRT_NEWTON_CONVERGED where FP32 and FP64 iterations part
-                                           ways (measured, not promised) */
+                                           ways (measured, not promised).  Traces in the persistent schedule
+                                           only, without ray_work */
 enum { BBRT_SEED_TRIANGLE = 0, BBRT_SEED_LEARNED = 1 };
 /* RayPath::Status, RayTracingDriver.hpp:69 */
 enum { BBRT_ESCAPED = 0, BBRT_TRAPPED = 1, BBRT_ERROR = 2 };
