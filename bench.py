@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--full-newton", choices=["fp32", "fp64"], default="fp32",
                     help="full form: Newton's first iterations in FP32 with the decisions and polish in FP64 "
                          "(BBRT_NEWTON_CONVERGED_FP32, the full form's precision split), or FP64 throughout")
+    ap.add_argument("--full-no-other-newton", dest="full_other_newton", action="store_false",
+                    help="full form: skip the learned-seed pass in the other Newton precision split")
     ap.add_argument("--full-no-triangle", dest="full_triangle", action="store_false",
                     help="full form: skip the triangle-seed run of the same workload")
     ap.add_argument("--full-scenes", type=int, default=512)
@@ -549,6 +551,13 @@ def full_form_leg(args):
     learned_value, learned_one = bounces / (te - ts), one_in_flight["value"]
     tri = full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, learned_value, learned_one,
                              mode) if args.full_triangle else None
+    # the other precision split on the same handle (learned seed): FP64 Newton throughout, or the FP32 start
+    other = L.NEWTON_CONVERGED if mode == L.NEWTON_CONVERGED_FP32 else L.NEWTON_CONVERGED_FP32
+    alt = full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, learned_value, learned_one,
+                             other, seed=L.SEED_LEARNED) if args.full_other_newton else None
+    if alt is not None:
+        alt["newton_mode"] = "converged" if other == L.NEWTON_CONVERGED else "converged, FP32 first iterations"
+        alt["note"] = "learned seed; learned_vs_triangle here is this configuration's learned value over this pass"
     return {
         "metric": METRIC, "value": bounces / (te - ts), "unit": "bounces/s", "n_gpus": 1, "steps": steps,
         "warmup": max(1, args.warmup), "ms_per_step": (te - ts) * 1e3 / steps, "higher_is_better": True,
@@ -565,7 +574,7 @@ def full_form_leg(args):
         "seed_fallback_per_step": int(ctr[L.CTR["seed_fallback"]]) // steps,
         "fallback_rate_per_scene": {"scenes": int(per.size), "min": float(per.min()), "median": float(np.median(per)),
                                     "mean": float(per.mean()), "max": float(per.max())} if per.size else None,
-        "oracle_check": check, "triangle_seed": tri,
+        "oracle_check": check, "triangle_seed": tri, "other_newton_mode": alt,
         "rays_by_status": status_counts(outs["status"]), "errors_by_kind": error_counts(outs["err_kind"]),
         "bounces_per_step": bounces // steps, "train": {"pairs": rep["n_pairs"], "steps": rep["steps"],
                                                          "test_loss": rep["test_loss"]},
@@ -574,24 +583,27 @@ def full_form_leg(args):
     }
 
 
-def full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, learned_value, learned_one, mode):
-    """The full form's workload with the triangle seed on the same handle, rays and bounce limit, timed the same
-    way (autotuned shape, hipGraph replays one at a time, then traces in flight): the learned path's ratio on
-    this workload.  Its records must equal the learned run's (the acceptance rule keeps every learned result one
-    the triangle path gives)."""
+def full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, learned_value, learned_one, mode,
+                       seed=None):
+    """The full form's workload on the same handle, rays and bounce limit with another seed mode (default the
+    triangle seed) or Newton mode, timed the same way (autotuned shape, hipGraph replays one at a time, then
+    traces in flight): the learned path's ratio on this workload.  Its records are compared with the learned
+    run's (the acceptance rule keeps every learned result one the triangle path gives)."""
     import numpy as np
     import torch
     from billardballraytracing3d_amd import _lib as L
 
+    seed = L.SEED_TRIANGLE if seed is None else seed
     keep = {k: outs[k].clone() for k in ("status", "err_kind", "bounces", "hit_patch", "hit_xyz")}
-    # the triangle seed writes no per-ray learned/fallback counts
-    outs = {k: v for k, v in outs.items() if k != "seed_stats"}
-    ptrs = {k: v.data_ptr() for k, v in outs.items()}
-    best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=mode, record_k=K, ray_begin=rb, stream=st)
+    if seed == L.SEED_TRIANGLE:   # the triangle seed writes no per-ray learned/fallback counts
+        outs = {k: v for k, v in outs.items() if k != "seed_stats"}
+        ptrs = {k: v.data_ptr() for k, v in outs.items()}
+    best, rows = gb.autotune(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=mode, record_k=K, ray_begin=rb, stream=st,
+                             seed=seed)
 
     def step():
         return gb.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=mode, record_k=K, ray_begin=rb,
-                               stream=st, flags=L.FLAG_GRAPH, **best)
+                               stream=st, flags=L.FLAG_GRAPH, seed=seed, **best)
 
     for _ in range(max(1, args.warmup)):
         step()
@@ -607,7 +619,8 @@ def full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, le
     value, ms = one, (te - ts) * 1e3 / steps
     if args.full_in_flight > 1:
         dt, cd, _ = run_in_flight(gb, to, td, n, mb, outs, steps, max(1, args.warmup), args.full_in_flight,
-                                  ctr // steps, dict(mode=mode, record_k=K, ray_begin=rb, flags=group_flag(args), **best))
+                                  ctr // steps, dict(mode=mode, record_k=K, ray_begin=rb, flags=group_flag(args),
+                                                     seed=seed, **best))
         value, ms = int(cd[:, L.CTR["bounces"]].sum()) / dt, dt * 1e3 / steps
     slot = torch.arange(K, device=outs["bounces"].device)[None, :]
     valid = slot < torch.clamp(keep["bounces"], max=K)[:, None].to(slot.dtype)
@@ -616,8 +629,8 @@ def full_form_triangle(args, gb, to, td, n, mb, K, rb, outs, ptrs, st, steps, le
     diff = (outs["hit_xyz"] - keep["hit_xyz"]).abs().amax(dim=2)
     max_diff = float(torch.where(valid, diff, torch.zeros_like(diff)).max().item())
     calls = int(ctr[L.CTR["newton_calls"]])
-    _progress(f"full form, triangle seed: {value:.4g} bounces/s ({args.full_in_flight} in flight), "
-              f"{one:.4g} one at a time")
+    _progress(f"full form, {'triangle' if seed == L.SEED_TRIANGLE else 'learned'} seed, newton mode {mode}: "
+              f"{value:.4g} bounces/s ({args.full_in_flight} in flight), {one:.4g} one at a time")
     return {"value": value, "ms_per_step": ms, "one_in_flight": {"value": one, "ms_per_step": (te - ts) * 1e3 / steps},
             "autotuned": best, "learned_vs_triangle": learned_value / value,
             "learned_vs_triangle_one_in_flight": learned_one / one,
