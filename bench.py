@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--waves-per-eu", type=int, default=0)
     ap.add_argument("--wavefront", action="store_true", help="per-bounce launches instead of the persistent kernel")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seed leg")
+    ap.add_argument("--no-seed-first", action="store_true",
+                    help="learned leg: skip the network-first pass (BBRT_FLAG_SEED_FIRST)")
     ap.add_argument("--train-samples", type=int, default=20_000_000)
     ap.add_argument("--train-epochs", type=int, default=10)
     ap.add_argument("--train-batch", type=int, default=8192)
@@ -695,6 +697,42 @@ def learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value_tri
     mismatch = int((~same).sum().item())
     kernel_ms = kms[0]
     mlp_tflops = evals * MLP_FLOPS_PER_WAVE_EVAL / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
+
+    def records_vs_triangle():
+        same = (outs["status"] == ref["status"]) & (outs["err_kind"] == ref["err_kind"]) & \
+            (outs["bounces"] == ref["bounces"])
+        if kk > 0:
+            slot = torch.arange(kk, device=outs["bounces"].device)[None, :]
+            valid = slot < torch.clamp(ref["bounces"], max=kk)[:, None].to(slot.dtype)
+            same &= torch.all((outs["hit_patch"][:, :kk] == ref["hit_patch"]) | ~valid, dim=1)
+        return int((~same).sum().item())
+
+    # the network-first form (BBRT_FLAG_SEED_FIRST): stage 1 only where the network seed's Newton is not decisive
+    first = None
+    if not args.no_seed_first:
+        def fstep():
+            return gs.trace_device(to.data_ptr(), td.data_ptr(), n, mb, ptrs, mode=L.NEWTON_CONVERGED, record_k=K,
+                                   flags=flags | L.FLAG_SEED_FIRST, stream=stream.cuda_stream, block=args.block,
+                                   waves_per_eu=args.waves_per_eu, seed=L.SEED_LEARNED, newton_batch=args.newton_batch)
+        for _ in range(args.warmup):
+            fstep()
+        torch.cuda.synchronize()
+        fctr = np.zeros(L.NCOUNTERS, np.int64)
+        fs = time.perf_counter()
+        for _ in range(args.steps):
+            c, _ = fstep()
+            fctr += c
+        torch.cuda.synchronize()
+        fe = time.perf_counter()
+        fv = int(fctr[L.CTR["bounces"]]) / (fe - fs)
+        first = {"value": fv, "ms_per_step": (fe - fs) * 1e3 / args.steps,
+                 "vs_triangle": fv / value_tri if value_tri else None, "vs_learned_stage1_first": fv / value,
+                 "per_step": {k: int(fctr[L.CTR[k]]) // args.steps for k in
+                              ("seed_learned", "seed_miss", "seed_fallback", "mt_tests", "newton_iters", "bounces")},
+                 "rays_differing_from_triangle_records": records_vs_triangle(), "bounces_compared": kk,
+                 "note": "network seed to Newton first; in-domain root = hit, converged outside the patch by more "
+                         "than the margin = miss, else stage 1 + triangle seed; records measured against the "
+                         "triangle-seed run (not promised equal)"}
     return {
         "value": value, "unit": "bounces/s", "ms_per_step": (te - ts) * 1e3 / args.steps,
         "vs_triangle": value / value_tri if value_tri else None,
@@ -709,6 +747,7 @@ def learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value_tri
                          "unit": "TFLOP/s", "frac": mlp_tflops / BF16_PEAK_TFLOPS},
         "check_vs_triangle": {"rays": n, "bounces_compared": kk, "rays_with_mismatch": mismatch,
                               "max_abs_hit_diff": max_diff},
+        "seed_first": first,
         "train": {"pairs": rep["n_pairs"], "steps": rep["steps"], "batch": args.train_batch,
                   "epochs": args.train_epochs, "first_loss": rep["first_loss"], "final_loss": rep["final_loss"],
                   "val_loss": rep["val_loss"], "test_loss": rep["test_loss"], "gen_ms": rep["gen_ms"],

@@ -21,11 +21,12 @@ ERR_NONE, ERR_SINGULAR, ERR_NOCONV, ERR_DOMAIN, ERR_BACKFACE, ERR_ZERO_DIR, ERR_
 FLAG_NO_CULL, FLAG_COUNT_DIRECT_HIT, FLAG_TIME_KERNELS, FLAG_WAVEFRONT, FLAG_GRAPH = 1, 2, 4, 8, 16
 FLAG_STAGE1_WAVE, FLAG_STAGE1_LANE = 32, 64
 FLAG_GROUP_RAYS = 128
+FLAG_SEED_FIRST = 256
 SCENE_COMPACT, SCENE_DEVICE_BUILD = 1, 2
-NCOUNTERS = 16
+NCOUNTERS = 20
 CTR = dict(rays=0, bounces=1, bvh_node_tests=2, candidates=3, cull_box_tests=4, mt_tests=5, newton_calls=6,
            newton_iters=7, newton_flops=8, polish_fail=9, launches=10, seed_learned=11, seed_fallback=12,
-           mlp_evals=13, mlp_lanes=14, newton_iters_fp32=15)
+           mlp_evals=13, mlp_lanes=14, newton_iters_fp32=15, seed_miss=16)
 MLP_NIN = 51
 MLP_NPARAMS = 13520
 

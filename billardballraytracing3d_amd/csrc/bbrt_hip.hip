@@ -215,7 +215,9 @@ enum : int { kPhTrav = 1, kPhLeaf = 2, kPhDone = 3, kPhSeed = 4 };
 // kept only under the acceptance rule (converged, polished, within
 // mlp::kSeedWindow of the triangle seed), else Newton runs again from the
 // triangle seed and its outcome stands.
-template <int W, bool WORK, bool LEARNED, bool COMPACT, bool F32 = false>
+// FIRST (with LEARNED, BBRT_FLAG_SEED_FIRST): the network seed goes to Newton before stage 1, which runs only
+// when that Newton neither converges inside the patch nor clearly outside it.
+template <int W, bool WORK, bool LEARNED, bool COMPACT, bool F32 = false, bool FIRST = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_t* __restrict__ n_live_p,
           int32_t* head, Outs O, int max_bounces, int record_k, int mode, uint32_t flags, unsigned long long* ctr,
@@ -420,7 +422,14 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         const unsigned long long in_leaf = __ballot(r >= 0 && phase == kPhLeaf);
         const int coop_max = (flags & BBRT_FLAG_STAGE1_WAVE) ? 64
                              : ((flags & BBRT_FLAG_STAGE1_LANE) ? 0 : (LEARNED ? kCoopLanesLearned : kCoopLanes));
-        if (COMPACT || (int)__popcll(in_leaf) > coop_max) {
+        if constexpr (FIRST) {
+            // straight to the network and Newton with the leaf's next entered object
+            if (r >= 0 && phase == kPhLeaf) {
+                obj = S.leaf[leaf_start + q];
+                c.candidates++;
+                phase = kPhSeed;
+            }
+        } else if (COMPACT || (int)__popcll(in_leaf) > coop_max) {
             if (r >= 0 && phase == kPhLeaf) {
                 for (;;) {
                     obj = S.leaf[leaf_start + q];
@@ -468,13 +477,30 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 if (need) {
                     c.mlp_lanes++;
                     h = newton_any<F32>(S, obj, o, d, ul, vl, tl, mode, c);
-                    const bool ok = h.status == 1 && h.polish_ok && fabs(h.u - sd.u) <= mlp::kSeedWindow &&
-                                    fabs(h.v - sd.v) <= mlp::kSeedWindow;
-                    if (ok) {
-                        c.seed_learned++;
+                    if constexpr (FIRST) {
+                        constexpr double lim = 1.0 + BBRT_SEED_FIRST_MISS_MARGIN;
+                        if (h.status == 1 && h.polish_ok) {
+                            c.seed_learned++;
+                        } else if (h.status == -bbrt::kErrDomain && (fabs(h.u) > lim || fabs(h.v) > lim)) {
+                            c.seed_miss++;
+                            h = PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+                        } else {
+                            // stage 1 and the triangle seed decide (SurfacePatch.hpp:253-303)
+                            c.seed_fallback++;
+                            c.candidates--;   // stage1_seed counts the candidate again
+                            sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
+                            h = sd.have ? newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c)
+                                        : PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
+                        }
                     } else {
-                        c.seed_fallback++;
-                        h = newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
+                        const bool ok = h.status == 1 && h.polish_ok && fabs(h.u - sd.u) <= mlp::kSeedWindow &&
+                                        fabs(h.v - sd.v) <= mlp::kSeedWindow;
+                        if (ok) {
+                            c.seed_learned++;
+                        } else {
+                            c.seed_fallback++;
+                            h = newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
+                        }
                     }
                 }
             } else {
@@ -997,6 +1023,10 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         throw std::runtime_error("bbrt_trace: BBRT_NEWTON_CONVERGED_FP32 runs in the persistent schedule only");
     if (f32 && out && out->ray_work)
         throw std::runtime_error("bbrt_trace: ray_work is not recorded with BBRT_NEWTON_CONVERGED_FP32");
+    const bool first = (p->flags & BBRT_FLAG_SEED_FIRST) != 0;
+    if (first && (!learned || f32 || (p->flags & BBRT_FLAG_WAVEFRONT)))
+        throw std::runtime_error("bbrt_trace: BBRT_FLAG_SEED_FIRST needs the learned seed, BBRT_NEWTON_CONVERGED and "
+                                 "the persistent schedule");
     unsigned grid = 1;
     const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
     const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch;
@@ -1005,7 +1035,16 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         const bool work = O.ray_work != nullptr;
         if (wpe != 2 && wpe != 3 && wpe != 4) throw std::runtime_error("bbrt_trace: waves_per_eu must be 0, 2, 3 or 4");
         if (s->compact && work) throw std::runtime_error("bbrt_trace: ray_work is not recorded on a compact scene");
-        if (f32) {
+        if (first) {
+            if (s->compact)
+                kern = wpe == 2 ? &k_persist<2, false, true, true, false, true>
+                                : (wpe == 3 ? &k_persist<3, false, true, true, false, true>
+                                            : &k_persist<4, false, true, true, false, true>);
+            else
+                kern = wpe == 2 ? &k_persist<2, false, true, false, false, true>
+                                : (wpe == 3 ? &k_persist<3, false, true, false, false, true>
+                                            : &k_persist<4, false, true, false, false, true>);
+        } else if (f32) {
             if (s->compact && learned)
                 kern = wpe == 2 ? &k_persist<2, false, true, true, true>
                                 : (wpe == 3 ? &k_persist<3, false, true, true, true> : &k_persist<4, false, true, true, true>);

@@ -107,6 +107,7 @@ struct Counters {
         bounces;
     uint32_t seed_learned, seed_fallback, mlp_evals, mlp_lanes;   // learned-seed mode only
     uint32_t newton_iters32;                                      // BBRT_NEWTON_CONVERGED_FP32: FP32 iterations
+    uint32_t seed_miss;                                           // BBRT_FLAG_SEED_FIRST: misses without stage 1
 };
 
 // MollerTrumbore.hpp:32-64 (t only; the point and normal are unused by SurfacePatch).
@@ -852,7 +853,7 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
     }
     if (u < -1.0 - kDomainTol || u > 1.0 + kDomainTol || v < -1.0 - kDomainTol || v > 1.0 + kDomainTol) {
         c.newton_iters += iters; c.newton_flops += iters * I_it;
-        r.status = -bbrt::kErrDomain; r.iters = iters; return r;
+        r.status = -bbrt::kErrDomain; r.iters = iters; r.u = u; r.v = v; return r;
     }
     if (mode != BBRT_NEWTON_REF) {
         bool ok = false;
@@ -1072,15 +1073,16 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ __forceinline__ void flush_counters(const Counters& c, unsigned long long* ctr) {
-    const unsigned long long v[14] = { c.bvh_tests, c.candidates, c.cull_tests, c.mt_tests, c.newton_calls,
+    const unsigned long long v[15] = { c.bvh_tests, c.candidates, c.cull_tests, c.mt_tests, c.newton_calls,
                                        c.newton_iters, c.newton_flops, c.polish_fail, c.bounces, c.seed_learned,
-                                       c.seed_fallback, c.mlp_evals, c.mlp_lanes, c.newton_iters32 };  // widened
-    const int slot[14] = { bbrt::kCtrBvh, bbrt::kCtrCand, bbrt::kCtrCull, bbrt::kCtrMt, bbrt::kCtrNewtonCalls,
+                                       c.seed_fallback, c.mlp_evals, c.mlp_lanes, c.newton_iters32,
+                                       c.seed_miss };  // widened
+    const int slot[15] = { bbrt::kCtrBvh, bbrt::kCtrCand, bbrt::kCtrCull, bbrt::kCtrMt, bbrt::kCtrNewtonCalls,
                            bbrt::kCtrNewtonIters, bbrt::kCtrNewtonFlops, bbrt::kCtrPolishFail, bbrt::kCtrBounces,
                            BBRT_CTR_SEED_LEARNED, BBRT_CTR_SEED_FALLBACK, BBRT_CTR_MLP_EVALS, BBRT_CTR_MLP_LANES,
-                           BBRT_CTR_NEWTON_ITERS_FP32 };
+                           BBRT_CTR_NEWTON_ITERS_FP32, BBRT_CTR_SEED_MISS };
 #pragma unroll
-    for (int q = 0; q < 14; ++q) {
+    for (int q = 0; q < 15; ++q) {
         const unsigned long long s = wave_sum(v[q]);
         if (__lane_id() == 0 && s) atomicAdd(ctr + slot[q], s);
     }

@@ -25,7 +25,7 @@ class TraceResult:
     hit_xyz: np.ndarray      # float64 [n, K, 3]
     iters: np.ndarray        # uint8 [n, K]
     final_od: np.ndarray     # float64 [n, 6]
-    counters: np.ndarray     # int64 [16]
+    counters: np.ndarray     # int64 [NCOUNTERS]
     hit_count: Optional[np.ndarray] = None
     ray_work: Optional[np.ndarray] = None   # uint32 [n, 4]
     seed_stats: Optional[np.ndarray] = None # uint32 [n, 2]: learned roots kept / fallbacks

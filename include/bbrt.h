@@ -79,6 +79,15 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
        BBRT_FLAG_STAGE1_LANE = 64u,     /* ... every stage 1 per lane (default: wave-cooperative when at most
                                            16 lanes of a wave need one, 8 with the learned seed); results and
                                            counters do not change */
+       BBRT_FLAG_SEED_FIRST = 256u,     /* learned seed, network first (opt-in, parity measured, not promised):
+                                           each candidate patch goes to Newton from the network's seed without
+                                           stage 1.  A root that converges inside the patch (classification and
+                                           polish) is the hit; one that converges outside the domain by more
+                                           than BBRT_SEED_FIRST_MISS_MARGIN in u or v is a miss; anything else
+                                           runs stage 1 and the triangle seed, whose outcome stands.  Stage 1
+                                           is then skipped for most candidates, but a patch the reference's
+                                           stage 1 misses can be hit and a farther root can be taken, so the
+                                           records can differ from the reference's */
        BBRT_FLAG_GROUP_RAYS = 128u };   /* persistent kernel: take the live rays grouped by where they pass
                                            their scene (Morton order of 32^3 cells of the root box; sub-scene
                                            major) instead of in input order.  Each wave then traces rays that
@@ -86,6 +95,8 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
                                            (bbrt_trace_device_async); one trace at a time it can be slower,
                                            since the expensive rays then come together.  Results and counters
                                            do not change */
+
+#define BBRT_SEED_FIRST_MISS_MARGIN 0.25
 
 typedef struct {
     int32_t newton_mode;   /* BBRT_NEWTON_* */
@@ -145,7 +156,10 @@ enum { BBRT_CTR_RAYS = 0, BBRT_CTR_BOUNCES = 1, BBRT_CTR_BVH_NODE_TESTS = 2, BBR
        /* BBRT_NEWTON_CONVERGED_FP32: FP32 Newton iterations (not in NEWTON_ITERS or NEWTON_FLOPS, which
           count the FP64 work) */
        BBRT_CTR_NEWTON_ITERS_FP32 = 15,
-       BBRT_NCOUNTERS = 16 };
+       /* BBRT_FLAG_SEED_FIRST: candidates the network seed's Newton declared missed (converged outside the
+          patch by more than BBRT_SEED_FIRST_MISS_MARGIN), without stage 1 */
+       BBRT_CTR_SEED_MISS = 16,
+       BBRT_NCOUNTERS = 20 };
 
 /* Builds a scene: per patch the barycentric weights, centroid, node box,
  * 100x100 coarse grid, per-cell sagitta and normal and the inflated box
