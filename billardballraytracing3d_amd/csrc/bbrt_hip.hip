@@ -476,10 +476,12 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 if (lane == 0) c.mlp_evals++;
                 if (need) {
                     c.mlp_lanes++;
-                    h = newton_any<F32>(S, obj, o, d, ul, vl, tl, mode, c);
+                    h = newton_any<F32, FIRST>(S, obj, o, d, ul, vl, tl, mode, c);
                     if constexpr (FIRST) {
                         constexpr double lim = 1.0 + BBRT_SEED_FIRST_MISS_MARGIN;
-                        if (h.status == 1 && h.polish_ok) {
+                        // a root behind the origin or at it (the point the ray was nudged off) is no stage-1
+                        // hit: Moller-Trumbore rejects t < 1e-9 (MollerTrumbore.hpp:54-55)
+                        if (h.status == 1 && h.polish_ok && h.t >= kMtEps) {
                             c.seed_learned++;
                         } else if (h.status == -bbrt::kErrDomain && (fabs(h.u) > lim || fabs(h.v) > lim)) {
                             c.seed_miss++;

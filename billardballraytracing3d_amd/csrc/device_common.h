@@ -769,7 +769,12 @@ __device__ __forceinline__ bool solve3(const double J[3][3], const V3& b, double
 // Newton on (u, v, t), SurfacePatch.hpp:593-660; the converged modes add the polish.  F32: the kernel carries
 // the FP32 start of BBRT_NEWTON_CONVERGED_FP32 (a separate instantiation, so the FP64-only kernels keep their
 // registers), for the 3x3 and 5x5 node sets; other node counts run FP64 throughout.
-template <int N, bool F32 = false>
+// PROBE: the network-first form's Newton from the network seed -- at most kProbeIters iterations, and it gives
+// up (a no-convergence result) as soon as the iterate is farther than kProbeBox from the patch in u or v, so a
+// candidate the ray misses costs a few iterations before stage 1 decides it.
+constexpr int kProbeIters = 6;
+constexpr double kProbeBox = 3.0;
+template <int N, bool F32 = false, bool PROBE = false>
 __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const V3& d, double u, double v, double t,
                        int mode, Counters& c) {
     constexpr int L = N > 0 ? N : bbrt::kMaxNodes1D;
@@ -836,7 +841,7 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
         c.newton_iters32 += it32;
         if (ok32) { u = fu; v = fv; t = ft; }
     }
-    for (int it = 0; it < kNewtonMaxIters; ++it) {
+    for (int it = 0; it < (PROBE ? kProbeIters : kNewtonMaxIters); ++it) {
         double delta[3];
         if (!step(delta)) {
             c.newton_iters += iters; c.newton_flops += iters * I_it;
@@ -846,6 +851,8 @@ __device__ PHit newton(const DevScene& S, const PatchMeta& M, const V3& o, const
         ++iters;
         step_norm = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
         if (step_norm < kNewtonTol) break;
+        if constexpr (PROBE)
+            if (!(fabs(u) <= kProbeBox && fabs(v) <= kProbeBox)) break;
     }
     if (step_norm >= kNewtonTol) {
         c.newton_iters += iters; c.newton_flops += iters * I_it;
@@ -1049,13 +1056,13 @@ __device__ __forceinline__ Seed seed_of(const Stage1& best) {
 }
 
 // newton_refine from (u0, v0, t0) on patch k, specialised on the node count.
-template <bool F32 = false>
+template <bool F32 = false, bool PROBE = false>
 __device__ __forceinline__ PHit newton_any(const DevScene& S, int64_t k, const V3& o, const V3& d, double u0, double v0,
                                            double t0, int mode, Counters& c) {
     const PatchMeta M = S.meta[k];
-    if (M.imax == 3 && M.jmax == 3) return newton<3, F32>(S, M, o, d, u0, v0, t0, mode, c);
-    if (M.imax == 5 && M.jmax == 5) return newton<5, F32>(S, M, o, d, u0, v0, t0, mode, c);
-    return newton<0, F32>(S, M, o, d, u0, v0, t0, mode, c);
+    if (M.imax == 3 && M.jmax == 3) return newton<3, F32, PROBE>(S, M, o, d, u0, v0, t0, mode, c);
+    if (M.imax == 5 && M.jmax == 5) return newton<5, F32, PROBE>(S, M, o, d, u0, v0, t0, mode, c);
+    return newton<0, F32, PROBE>(S, M, o, d, u0, v0, t0, mode, c);
 }
 
 // SurfacePatch::intersect (SurfacePatch.hpp:253-303): stage 1, then Newton from its seed.
