@@ -334,7 +334,9 @@ __device__ __forceinline__ void weight_grad(const uint16_t (*dz)[kLdsPitch], con
         for (int it = 0; it < 4; ++it) {
             f32x4_t acc = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-            for (int sk = 0; sk < 4; ++sk) acc = mfma16(load_fm(dz, ot * 16, sk, lane), load_fm(act, it * 16, sk, lane), acc);
+            for (int sp = 0; sp < 2; ++sp)
+                acc = mfma32(load_fm(dz, ot * 16, 2 * sp, lane), load_fm(dz, ot * 16, 2 * sp + 1, lane),
+                             load_fm(act, it * 16, 2 * sp, lane), load_fm(act, it * 16, 2 * sp + 1, lane), acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i) part[(ot * 16 + 4 * q + i) * kH + it * 16 + j] = acc[i];
         }
@@ -357,8 +359,15 @@ __device__ __forceinline__ void back_layer(const u32x2_t* Wb, const u32x2_t (&dz
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
             f32x4_t acc = { 0.0f, 0.0f, 0.0f, 0.0f };
+            if constexpr (OT % 2 == 0) {
 #pragma unroll
-            for (int ot = 0; ot < OT; ++ot) acc = mfma16(Wb[(it * OT + ot) * 64 + lane], dzn[ot][st], acc);
+                for (int op = 0; op < OT / 2; ++op)
+                    acc = mfma32(Wb[(it * OT + 2 * op) * 64 + lane], Wb[(it * OT + 2 * op + 1) * 64 + lane],
+                                 dzn[2 * op][st], dzn[2 * op + 1][st], acc);
+            } else {
+#pragma unroll
+                for (int ot = 0; ot < OT; ++ot) acc = mfma16(Wb[(it * OT + ot) * 64 + lane], dzn[ot][st], acc);
+            }
             const f32x4_t g = gp[(it * 4 + st) * 64 + lane];
             dz[it][st] = pack4(acc[0] * g[0], acc[1] * g[1], acc[2] * g[2], acc[3] * g[3]);
         }
@@ -520,20 +529,60 @@ __global__ void k_mlp_reduce(const float* __restrict__ partial, int n_tiles, flo
     }
 }
 
-// Adam on the fp32 master weights (the pads have zero gradient and stay zero).
-__global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ m1, float* __restrict__ m2,
-                           const float* __restrict__ partial, int n_tiles, float lr, float bc1, float bc2,
-                           const float* __restrict__ loss_part, float inv_n, float* __restrict__ loss_hist) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < kNParams) {
-        float g = 0.0f;
-        for (int t = 0; t < n_tiles; ++t) g += partial[(size_t)t * kNParams + p];
-        const float a = 0.9f * m1[p] + 0.1f * g;
-        const float b = 0.999f * m2[p] + 0.001f * g * g;
-        m1[p] = a;
-        m2[p] = b;
-        P[p] -= lr * (a / bc1) / (sqrtf(b / bc2) + 1e-8f);
+// One weight's bf16 copy into every fragment slot that holds it (the inverse of
+// k_mlp_pack's gather): forward fragments of W1..W4, the W1 ray fragments, and
+// the transposed fragments of W2..W4 the backward pass reads.
+__device__ __forceinline__ void scatter_weight(uint16_t* __restrict__ F16, int p, uint16_t v) {
+    auto put = [&](int f, int lane, int slot) { F16[((size_t)f * 64 + lane) * 4 + slot] = v; };
+    if (p < kOffB1) {
+        const int r = (p - kOffW1) / kH, c = (p - kOffW1) % kH;
+        put(kFragFwd1 + (r / 16) * 4 + c / 16, ((c % 16) / 4) * 16 + r % 16, c % 4);
+        if (c < kRay) put(kFragRay1 + r / 16, (c / 4) * 16 + r % 16, c % 4);
+    } else if ((p >= kOffW2 && p < kOffB2) || (p >= kOffW3 && p < kOffB3)) {
+        const int L = p < kOffB2 ? 0 : 1;
+        const int off = L == 0 ? kOffW2 : kOffW3;
+        const int r = (p - off) / kH, c = (p - off) % kH;
+        put((L == 0 ? kFragFwd2 : kFragFwd3) + (r / 16) * 4 + c / 16, ((c % 16) / 4) * 16 + r % 16, c % 4);
+        put(kFragBwd2 + L * 16 + (c / 16) * 4 + r / 16, ((r % 16) / 4) * 16 + c % 16, r % 4);
+    } else if (p >= kOffW4 && p < kOffB4) {
+        const int r = (p - kOffW4) / kH, c = (p - kOffW4) % kH;
+        put(kFragFwd4 + c / 16, ((c % 16) / 4) * 16 + r, c % 4);
+        put(kFragBwd4 + c / 16, (r / 4) * 16 + c % 16, r % 4);
     }
+}
+
+// Adam on the fp32 master weights (the pads have zero gradient and stay zero)
+// fused with their repacking.  A workgroup owns kAdamParams parameters and
+// sums their tile gradients with kAdamGroups waves: wave w adds tiles w,
+// w + kAdamGroups, ... in order, then wave 0 adds the waves' sums in order, so
+// the step is deterministic.  Wave 0 then updates each parameter and writes its
+// bf16 value straight into the fragments (the slots no parameter owns -- the
+// ray fragments' zero columns -- keep what k_mlp_pack first wrote).
+constexpr int kAdamParams = 64, kAdamGroups = 8;
+
+__global__ void __launch_bounds__(kAdamParams * kAdamGroups)
+k_mlp_adam_pack(float* __restrict__ P, float* __restrict__ m1, float* __restrict__ m2, const float* __restrict__ partial,
+                int n_tiles, float lr, float bc1, float bc2, const float* __restrict__ loss_part, float inv_n,
+                float* __restrict__ loss_hist, uint16_t* __restrict__ F16) {
+    __shared__ float sums[kAdamGroups][kAdamParams];
+    const int pl = threadIdx.x % kAdamParams, grp = threadIdx.x / kAdamParams;
+    const int p = blockIdx.x * kAdamParams + pl;
+    float g = 0.0f;
+    if (p < kNParams)
+        for (int t = grp; t < n_tiles; t += kAdamGroups) g += partial[(size_t)t * kNParams + p];
+    sums[grp][pl] = g;
+    __syncthreads();
+    if (grp != 0 || p >= kNParams) return;
+    g = 0.0f;
+#pragma unroll
+    for (int w = 0; w < kAdamGroups; ++w) g += sums[w][pl];
+    const float a = 0.9f * m1[p] + 0.1f * g;
+    const float b = 0.999f * m2[p] + 0.001f * g * g;
+    m1[p] = a;
+    m2[p] = b;
+    const float w = P[p] - lr * (a / bc1) / (sqrtf(b / bc2) + 1e-8f);
+    P[p] = w;
+    scatter_weight(F16, p, (uint16_t)f2bf(w));
     if (p == 0 && loss_hist) {
         float s = 0.0f;
         for (int t = 0; t < n_tiles; ++t) s += loss_part[t];
@@ -873,9 +922,10 @@ int bbrt_mlp_train(bbrt_mlp m, bbrt_scene s, const bbrt_train_config* cfg, float
             const float bc1 = (float)(1.0 - std::pow(b1, (double)step)), bc2 = (float)(1.0 - std::pow(b2, (double)step));
             hipLaunchKernelGGL(k_mlp_train, dim3(tiles), dim3(64), 0, st, rows, tgt, first, B, 2.0f / (float)B, m->F, m->P,
                                part, lp);
-            hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kNParams, 256)), dim3(256), 0, st, m->P, m->m1, m->m2, part,
-                               tiles, (float)lr, bc1, bc2, lp, 1.0f / (float)B, hist + t);
-            hipLaunchKernelGGL(k_mlp_pack, dim3(blocks_for(kNFrags * 64, 256)), dim3(256), 0, st, m->P, m->F);
+            hipLaunchKernelGGL(k_mlp_adam_pack, dim3(blocks_for(kNParams, kAdamParams)), dim3(kAdamParams * kAdamGroups), 0,
+                               st, m->P, m->m1, m->m2,
+                               part, tiles, (float)lr, bc1, bc2, lp, 1.0f / (float)B, hist + t,
+                               reinterpret_cast<uint16_t*>(m->F));
         }
         check(hipGetLastError(), "training launches");
         check(hipStreamSynchronize(st), "hipStreamSynchronize");

@@ -5,8 +5,10 @@
 // (ray, patch) pair.  The reference has no such seed; its seed is the centroid
 // of the closest stage-1 triangle (SurfacePatch.hpp:272-303, :499-510).
 //
-// Matrix work runs on the gfx950 matrix cores with v_mfma_f32_16x16x16_bf16:
-// bf16 operands, fp32 accumulation.  Every layer is computed transposed,
+// Matrix work runs on the gfx950 matrix cores, bf16 operands and fp32
+// accumulation: v_mfma_f32_16x16x32_bf16 for every 64-wide contraction (two
+// 16-wide K slices per instruction), v_mfma_f32_16x16x16_bf16 where only one
+// slice exists (layer 1's six ray columns, the output layer's transpose).  Every layer is computed transposed,
 // Z^T = W X^T, with the weights as the A operand and the activations as the B
 // operand.  The instruction's D layout (lane l holds rows 4(l/16)+i, column
 // l%16) is then exactly the B layout the next layer needs (lane l holds rows
@@ -62,6 +64,8 @@ namespace {
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 // float -> bfloat16, round to nearest even (inputs are finite)
 __device__ __forceinline__ uint32_t f2bf(float f) {
@@ -82,6 +86,17 @@ __device__ __forceinline__ u32x2_t pack4(float a, float b, float c, float d) {
 __device__ __forceinline__ f32x4_t mfma16(u32x2_t a, u32x2_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(bf16x4_t, a), __builtin_bit_cast(bf16x4_t, b),
                                                      c, 0, 0, 0);
+}
+
+// D = A B + C contracting over 32: two 16-wide K slices (a0, b0) and (a1, b1)
+// in mfma16's lane layout, issued as one gfx950 16x16x32 instruction.  Lane
+// group l/16 of A and of B carries the same eight K indices (4 of slice 0,
+// then 4 of slice 1), so the product is the sum over both slices whatever
+// order the instruction walks K in.
+__device__ __forceinline__ f32x4_t mfma32(u32x2_t a0, u32x2_t a1, u32x2_t b0, u32x2_t b1, f32x4_t c) {
+    const u32x4_t a = { a0.x, a0.y, a1.x, a1.y }, b = { b0.x, b0.y, b1.x, b1.y };
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c,
+                                                   0, 0, 0);
 }
 
 // GELU, tanh form: 0.5 z (1 + tanh(k0 (z + k1 z^3))), and its derivative.
@@ -113,7 +128,7 @@ __device__ __forceinline__ void hidden_layer(const u32x2_t* Wf, const float* b, 
         for (int st = 0; st < 4; ++st) {
             f32x4_t acc = bias;
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt) acc = mfma16(w[kt], in[kt][st], acc);
+            for (int kp = 0; kp < 2; ++kp) acc = mfma32(w[2 * kp], w[2 * kp + 1], in[2 * kp][st], in[2 * kp + 1][st], acc);
             float g0, g1, g2, g3;
             const float h0 = gelu_f(acc[0], g0), h1 = gelu_f(acc[1], g1), h2 = gelu_f(acc[2], g2),
                         h3 = gelu_f(acc[3], g3);
@@ -135,7 +150,7 @@ __device__ __forceinline__ void out_layer(const u32x2_t* Wf4, const float* b4, c
     for (int st = 0; st < 4; ++st) {
         f32x4_t acc = bias;
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = mfma16(w[kt], in[kt][st], acc);
+        for (int kp = 0; kp < 2; ++kp) acc = mfma32(w[2 * kp], w[2 * kp + 1], in[2 * kp][st], in[2 * kp + 1][st], acc);
         y[st] = acc;
     }
 }

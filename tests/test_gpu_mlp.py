@@ -117,6 +117,25 @@ def test_training_lowers_loss(trained):
     assert np.isfinite(net.params).all()
 
 
+def test_training_leaves_fragments_equal_to_a_fresh_pack(trained):
+    """Each Adam step writes the updated weights straight into the bf16 fragments (forward, transposed and
+    ray fragments).  After training they must be what packing the downloaded masters afresh gives: the
+    forward pass and the gradient (which reads the transposed fragments) agree bit for bit."""
+    net, _, _ = trained
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal((3000, M.N_IN)) * 0.5).astype(np.float32)
+    t = (rng.standard_normal((3000, 3)) * 0.1).astype(np.float32)
+    y1 = net.forward(x)
+    l1, g1 = net.loss_grad(x, t)
+    twin = T.Mlp(0)
+    twin.params = net.params
+    y2 = twin.forward(x)
+    l2, g2 = twin.loss_grad(x, t)
+    np.testing.assert_array_equal(y1, y2)
+    np.testing.assert_array_equal(g1, g2)
+    assert l1 == l2
+
+
 def test_learned_seed_lands_on_the_triangle_root(c1, trained):
     _, g, _, _, _, _ = c1
     ab = g.seed_ab(seed=99, first=0, n=20000)           # pairs never used in training (other seed)
