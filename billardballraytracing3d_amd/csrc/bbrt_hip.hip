@@ -429,7 +429,39 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 c.candidates++;
                 phase = kPhSeed;
             }
-        } else if (COMPACT || (int)__popcll(in_leaf) > coop_max) {
+        } else if constexpr (COMPACT) {
+            // compact scenes: each leaf lane walks its object's pyramid, and the cells the walks select are
+            // tested by the whole wave, 64 a pass; the walks resume until each lane's stage 1 hits (the lane
+            // parks for Newton) or its leaf ends
+            bool walking = r >= 0 && phase == kPhLeaf;
+            CompactWalk w{ 0u, 0u, 0u, 0, 0, 0, 0, 0, false };
+            Stage1 s1{ 0.0, 0, false };
+            const double widen = S.scenes[scn].cull_widen;
+            if (walking) {
+                obj = S.leaf[leaf_start + q];
+                c.candidates++;
+                compact_walk_init(S, obj, r32, cull && S.meta[obj].cull, w, c);
+            }
+            while (__ballot(walking)) {
+                int ti = 0, tj = 0;
+                uint32_t mc = 0;
+                bool tile = false;
+                if (walking) {
+                    for (;;) {
+                        if (compact_walk_next(S, obj, r32, w, ti, tj, mc, c)) { tile = true; break; }
+                        // this object's walk is over: its stage-1 outcome (SurfacePatch.hpp:253-303)
+                        const Seed s = seed_of(s1);
+                        if (s.have) { sd = s; phase = kPhSeed; walking = false; break; }
+                        if (!next_object()) { walking = false; break; }
+                        obj = S.leaf[leaf_start + q];
+                        c.candidates++;
+                        s1 = Stage1{ 0.0, 0, false };
+                        compact_walk_init(S, obj, r32, cull && S.meta[obj].cull, w, c);
+                    }
+                }
+                compact_cells_wave(S, tile, obj, ti, tj, mc, o, d, r32, widen, w.use_cull, s1, c);
+            }
+        } else if ((int)__popcll(in_leaf) > coop_max) {
             if (r >= 0 && phase == kPhLeaf) {
                 for (;;) {
                     obj = S.leaf[leaf_start + q];

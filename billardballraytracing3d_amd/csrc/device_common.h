@@ -539,61 +539,154 @@ __device__ __forceinline__ uint32_t qcell_mask(const DevScene& S, int64_t k, con
     return mc;
 }
 
-// Stage 1 of a compact scene.  Culled: the pyramid's super, block and tile
-// boxes select tiles, the quantized boxes select cells, each selected cell is
-// evaluated from the fields and its exact FP32 box rebuilt and tested as a
-// stored scene's, then Moller-Trumbore.  Not culled: every cell, tile by tile.
-// One loop yields the tiles in both cases, so the cell evaluation has a single
-// (inlined) site in the kernel.  Tiles are visited in the stored path's order;
-// ties between equal t are broken by the cell key, so the result does not
-// depend on the order anyway.
-__device__ void stage1_compact(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r, double widen,
-                               bool use_cull, Stage1& best, Counters& c) {
-    const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
-    const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
-    const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
-    uint32_t ms = 0, mb = 0, mt = 0;
-    int si = 0, sj = 0, bi = 0, bj = 0, t = 0;
+// The resumable walk of a compact patch's culling pyramid, one lane's state.
+// Culled: the pyramid's super, block and tile boxes select tiles, the
+// quantized boxes select the tile's cells.  Not culled: every cell, tile by
+// tile.  Tiles come in the stored path's order.
+struct CompactWalk {
+    uint32_t ms, mb, mt;
+    int si, sj, bi, bj, t;
+    bool use_cull;
+};
+
+__device__ __forceinline__ void compact_walk_init(const DevScene& S, int64_t k, const Ray32& r, bool use_cull,
+                                                  CompactWalk& w, Counters& c) {
+    w = CompactWalk{ 0u, 0u, 0u, 0, 0, 0, 0, 0, use_cull };
     if (use_cull) {
-        ms = group_mask<9, bbrt::kG9>(gs, r);
+        w.ms = group_mask<9, bbrt::kG9>(S.box_super + (size_t)k * bbrt::kSuperFloats, r);
         c.cull_tests += 9;
     }
+}
+
+// The next tile with a selected cell: true with the tile (ti, tj) and its
+// cell mask mc (bit 3 * (i % 3) + j % 3), false when the walk is over.
+__device__ __forceinline__ bool compact_walk_next(const DevScene& S, int64_t k, const Ray32& r, CompactWalk& w,
+                                                  int& ti, int& tj, uint32_t& mc, Counters& c) {
+    const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
+    const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
     for (;;) {
-        int ti, tj;
-        uint32_t mc;
-        if (use_cull) {
-            while (mt == 0) {
-                while (mb == 0) {
-                    if (ms == 0) return;
-                    const int sq = __ffs(ms) - 1;
-                    ms &= ms - 1;
-                    si = sq / 3; sj = sq % 3;
-                    mb = group_mask<16, bbrt::kG16>(gb + sq * bbrt::kG16, r);
+        if (w.use_cull) {
+            while (w.mt == 0) {
+                while (w.mb == 0) {
+                    if (w.ms == 0) return false;
+                    const int sq = __ffs(w.ms) - 1;
+                    w.ms &= w.ms - 1;
+                    w.si = sq / 3; w.sj = sq % 3;
+                    w.mb = group_mask<16, bbrt::kG16>(gb + sq * bbrt::kG16, r);
                     c.cull_tests += 16;
                 }
-                const int bq = __ffs(mb) - 1;
-                mb &= mb - 1;
-                bi = bbrt::kBlocksPerSuper * si + (bq >> 2); bj = bbrt::kBlocksPerSuper * sj + (bq & 3);
-                mt = group_mask<9, bbrt::kG9>(gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9, r);
+                const int bq = __ffs(w.mb) - 1;
+                w.mb &= w.mb - 1;
+                w.bi = bbrt::kBlocksPerSuper * w.si + (bq >> 2); w.bj = bbrt::kBlocksPerSuper * w.sj + (bq & 3);
+                w.mt = group_mask<9, bbrt::kG9>(gt + (w.bi * bbrt::kBlockN + w.bj) * bbrt::kG9, r);
                 c.cull_tests += 9;
             }
-            const int tq = __ffs(mt) - 1;
-            mt &= mt - 1;
-            ti = 3 * bi + tq / 3; tj = 3 * bj + tq % 3;
-            mc = qcell_mask(S, k, gt + (bi * bbrt::kBlockN + bj) * bbrt::kG9 + 6 * tq, ti, tj, r);
+            const int tq = __ffs(w.mt) - 1;
+            w.mt &= w.mt - 1;
+            ti = 3 * w.bi + tq / 3; tj = 3 * w.bj + tq % 3;
+            mc = qcell_mask(S, k, gt + (w.bi * bbrt::kBlockN + w.bj) * bbrt::kG9 + 6 * tq, ti, tj, r);
             c.cull_tests += 9;
         } else {
-            if (t == bbrt::kTileN * bbrt::kTileN) return;
-            ti = t / bbrt::kTileN; tj = t % bbrt::kTileN;
-            ++t;
+            if (w.t == bbrt::kTileN * bbrt::kTileN) return false;
+            ti = w.t / bbrt::kTileN; tj = w.t % bbrt::kTileN;
+            ++w.t;
             mc = 0x1FFu;
         }
+        if (mc) return true;
+    }
+}
+
+// A selected cell: evaluated from the fields, its exact FP32 box rebuilt and
+// tested as a stored scene's (culled walks), then Moller-Trumbore.
+__device__ __forceinline__ void compact_cell_tests(const DevScene& S, int64_t k, int i, int j, const V3& o,
+                                                   const V3& d, const Ray32& r, double widen, bool use_cull,
+                                                   Stage1& best, Counters& c) {
+    const CellData cd = compact_cell_any(S, k, i, j);
+    if (!use_cull || compact_cell_box(cd, widen, r)) cell_tests_data(cd, i, j, o, d, best, c);
+}
+
+// Stage 1 of a compact scene by one lane: its walk, each selected cell tested
+// as it comes.  Ties between equal t are broken by the cell key, so the result
+// does not depend on the order the cells are tested in.
+__device__ void stage1_compact(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r, double widen,
+                               bool use_cull, Stage1& best, Counters& c) {
+    CompactWalk w;
+    compact_walk_init(S, k, r, use_cull, w, c);
+    int ti, tj;
+    uint32_t mc;
+    while (compact_walk_next(S, k, r, w, ti, tj, mc, c)) {
         while (mc) {
             const int cq = __ffs(mc) - 1;
             mc &= mc - 1;
-            const int i = 3 * ti + cq / 3, j = 3 * tj + cq % 3;
-            const CellData cd = compact_cell_any(S, k, i, j);
-            if (!use_cull || compact_cell_box(cd, widen, r)) cell_tests_data(cd, i, j, o, d, best, c);
+            compact_cell_tests(S, k, 3 * ti + cq / 3, 3 * tj + cq % 3, o, d, r, widen, use_cull, best, c);
+        }
+    }
+}
+
+// k-th (0-based) set bit of m, which has more than k bits set.
+__device__ __forceinline__ int nth_bit(unsigned long long m, int k) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const int mid = lo + step;
+        if ((int)__popcll(m & ((1ull << mid) - 1ull)) <= k) lo = mid;
+    }
+    return lo;
+}
+
+// The cells the lanes' walks selected, tested by the whole wave (every lane
+// calls this; `have` marks the lanes with a tile).  The wave's list holds the
+// lanes' cells in lane order, each lane's in bit order; lane e of a pass takes
+// list entry e, tests that cell of its owner's patch for its owner's ray, and
+// the owners fold the entries' results into their stage-1 best.  The cells
+// tested and the counts are the per-lane walk's, and the best is the minimum
+// of (t, key), so the result is the per-lane walk's too; but a cell costs one
+// lane of a 64-wide pass instead of a pass of its own.
+__device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have, int32_t k, int ti, int tj,
+                                                   uint32_t mc, const V3& o, const V3& d, const Ray32& r,
+                                                   double widen, bool use_cull, Stage1& best, Counters& c) {
+    const int lane = __lane_id();
+    const int cnt = have ? __popc(mc) : 0;
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    const int start = incl - cnt;
+    for (int p0 = 0; p0 < total; p0 += 64) {
+        const int e = p0 + lane;
+        int own = 0;   // lanes whose cells all come before entry e: its owner
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (__shfl(incl, own + step - 1) <= e) own += step;
+        const int32_t kk = __shfl(k, own);
+        const int oti = __shfl(ti, own), otj = __shfl(tj, own), ost = __shfl(start, own);
+        const uint32_t omc = (uint32_t)__shfl((int)mc, own);
+        const V3 oo = { __shfl(o.x, own), __shfl(o.y, own), __shfl(o.z, own) };
+        const V3 od = { __shfl(d.x, own), __shfl(d.y, own), __shfl(d.z, own) };
+        const Ray32 orr = { __shfl(r.ox, own), __shfl(r.oy, own), __shfl(r.oz, own),
+                            __shfl(r.ix, own), __shfl(r.iy, own), __shfl(r.iz, own) };
+        const double ow = __shfl(widen, own);
+        const bool ouc = __shfl(use_cull ? 1 : 0, own) != 0;
+        Stage1 ib{ 0.0, 0, false };
+        if (e < total) {
+            const int cq = nth_bit(omc, e - ost);
+            compact_cell_tests(S, kk, 3 * oti + cq / 3, 3 * otj + cq % 3, oo, od, orr, ow, ouc, ib, c);
+        }
+        // each owner's entries of this pass: list entries start .. start + cnt - 1
+        for (int u = 0; u < 9; ++u) {
+            if (__ballot(u < cnt) == 0) break;   // past every lane's last entry
+            const int src = start + u - p0;
+            const bool take = u < cnt && src >= 0 && src < 64;
+            const int sl = src & 63;
+            const double t2 = __shfl(ib.t, sl);
+            const int k2 = __shfl(ib.key, sl);
+            const bool h2 = __shfl(ib.have ? 1 : 0, sl) != 0;
+            if (take && h2 && (!best.have || t2 < best.t || (t2 == best.t && k2 < best.key))) {
+                best.have = true; best.t = t2; best.key = k2;
+            }
         }
     }
 }
@@ -909,16 +1002,6 @@ __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const 
     return seed_of(best);
 }
 
-// k-th (0-based) set bit of m, which has more than k bits set.
-__device__ __forceinline__ int nth_bit(unsigned long long m, int k) {
-    int lo = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-        const int mid = lo + step;
-        if ((int)__popcll(m & ((1ull << mid) - 1ull)) <= k) lo = mid;
-    }
-    return lo;
-}
 
 // Stage 1 of one (ray, patch) pair by the whole wave (every lane calls it with
 // the same arguments).  The pyramid is walked level by level, each level's

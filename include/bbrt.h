@@ -78,7 +78,8 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
        BBRT_FLAG_STAGE1_WAVE = 32u,     /* persistent kernel, stored scenes: every stage 1 wave-cooperative */
        BBRT_FLAG_STAGE1_LANE = 64u,     /* ... every stage 1 per lane (default: wave-cooperative when at most
                                            16 lanes of a wave need one, 8 with the learned seed); results and
-                                           counters do not change */
+                                           counters do not change.  Compact scenes ignore both: each lane
+                                           walks its own pyramid and the wave tests the selected cells */
        BBRT_FLAG_SEED_FIRST = 256u,     /* learned seed, network first (opt-in, parity measured, not promised):
                                            each candidate patch goes to Newton from the network's seed without
                                            stage 1.  A root that converges inside the patch (classification and

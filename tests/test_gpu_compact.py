@@ -75,3 +75,28 @@ def test_device_build_equals_host_build():
     _same(a.trace(o, d, 25, ray_begin=rb), b.trace(o, d, 25, ray_begin=rb))
     sc, o1, d1, mb = S.two_spheres_plane_source()
     _same(T.GpuScene.batch([sc], compact=True, device_build=True).trace(o1, d1, mb), T.GpuScene(sc).trace(o1, d1, mb))
+
+
+
+def test_wave_cell_tests_on_mixed_orders_and_brute_force():
+    """Compact stage 1 tests the cells the lanes' walks select wave-wide: a lane's cells can be tested by
+    other lanes, for patches of other orders, and a brute-force walk (every tile) spans many passes of the
+    wave's list.  Records and work counters equal the stored scene's, culled and brute force, with the
+    triangle and with the learned seed (whose count of wave-level network passes follows the schedule)."""
+    scenes, o, d, rb = S.batch_config(seed=8, n_scenes=4, patch_lo=80, patch_hi=700, rays_lo=400, rays_hi=2500)
+    cp = T.GpuScene.batch(scenes, compact=True, device_build=True)
+    st = T.GpuScene.batch(scenes)
+    keep = np.arange(L.NCOUNTERS) != L.CTR["mlp_evals"]
+    for learned in (False, True):
+        seed = L.SEED_TRIANGLE
+        if learned:
+            net = T.Mlp(4)
+            cp.set_mlp(net)
+            st.set_mlp(net)
+            seed = L.SEED_LEARNED
+        for fl in (0, L.FLAG_NO_CULL):
+            a = cp.trace(o, d, 25, ray_begin=rb, mode=L.NEWTON_CONVERGED, seed=seed, flags=fl)
+            b = st.trace(o, d, 25, ray_begin=rb, mode=L.NEWTON_CONVERGED, seed=seed, flags=fl)
+            _same(a, b)
+            np.testing.assert_array_equal(a.counters[keep], b.counters[keep])
+            assert a.counter("bounces") > 0
