@@ -90,8 +90,11 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
                                            stage 1 misses can be hit and a farther root can be taken, so the
                                            records can differ from the reference's */
        BBRT_FLAG_GROUP_RAYS = 128u };   /* persistent kernel: take the live rays grouped by where they pass
-                                           their scene (Morton order of 32^3 cells of the root box; sub-scene
-                                           major) instead of in input order.  Each wave then traces rays that
+                                           their scene (Morton order of the cells of the root box: 32^3 for a
+                                           one-scene handle, 4^3 per sub-scene of a batch handle, sub-scene
+                                           major; above 2^22 bins, i.e. more than 65,536 sub-scenes, the flag
+                                           is ignored and the queue keeps input order) instead of in input
+                                           order.  Each wave then traces rays that
                                            pass the same part of the scene.  Pays with several traces in flight
                                            (bbrt_trace_device_async); one trace at a time it can be slower,
                                            since the expensive rays then come together.  Results and counters
