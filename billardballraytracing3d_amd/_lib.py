@@ -38,7 +38,8 @@ EXPORTS = ("bbrt_scene_create", "bbrt_scene_destroy", "bbrt_scene_info", "bbrt_s
            "bbrt_scene_set_mlp", "bbrt_mlp_seed_ab", "bbrt_scene_create_batch", "bbrt_scene_batch_info",
            "bbrt_trace_batch", "bbrt_trace_batch_device", "bbrt_autotune", "bbrt_scene_create_ex",
            "bbrt_trace_device_async", "bbrt_trace_device_noalloc", "bbrt_moller_trumbore", "bbrt_triangles_create",
-           "bbrt_triangles_destroy", "bbrt_triangles_intersect", "bbrt_scene_intersect", "bbrt_bounce_map")
+           "bbrt_triangles_destroy", "bbrt_triangles_intersect", "bbrt_scene_intersect", "bbrt_bounce_map",
+           "bbrt_build_bvh_gpu")
 NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _d = C.POINTER(C.c_double)
@@ -122,6 +123,8 @@ def lib():
         L.bbrt_prepare_patches.argtypes = [C.POINTER(PatchSet), _d, _d, _d, _d, _d]
         L.bbrt_build_bvh.argtypes = [_d, _d, C.c_int64, C.c_int32, _d, _i32, _i32, _i32, _u8, _i32]
         L.bbrt_build_bvh.restype = C.c_int64
+        L.bbrt_build_bvh_gpu.argtypes = [_d, _d, C.c_int64, C.c_int32, _d, _i32, _i32, _i32, _u8, _i32]
+        L.bbrt_build_bvh_gpu.restype = C.c_int64
         _f = C.POINTER(C.c_float)
         L.bbrt_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
         L.bbrt_mlp_destroy.argtypes = [C.c_void_p]

@@ -30,15 +30,31 @@ def _run(cmd, verbose):
         print(r.stdout)
 
 
+def _run_all(cmds, verbose):
+    """Runs the compile commands side by side; raises on the first that fails."""
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd), flush=True)
+    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True) for cmd in cmds]
+    outs = [p.communicate()[0] for p in procs]
+    for cmd, p, out in zip(cmds, procs, outs):
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            raise RuntimeError(f"build step failed ({p.returncode}): {' '.join(cmd)}")
+        elif verbose and out.strip():
+            print(out)
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     o_host = os.path.join(BUILD, "scene_build.o")
-    _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", o_host], verbose)
+    cmds = [["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, "scene_build.cpp"), "-o", o_host]]
     objs = []
-    for name in ("bbrt_hip", "bbrt_mlp", "bbrt_objects"):
+    for name in ("bbrt_hip", "bbrt_mlp", "bbrt_objects", "bbrt_bvh"):
         o = os.path.join(BUILD, name + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, name + ".hip"), "-o", o], verbose)
+        cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, name + ".hip"), "-o", o])
         objs.append(o)
+    _run_all(cmds, verbose)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + [o_host], verbose)
     os.replace(tmp, LIB)

@@ -37,6 +37,7 @@
 
 #include "mlp_device.h"
 #include "runtime.h"
+#include "bvh_device.h"
 
 thread_local std::string bbrt::g_last_error;
 
@@ -1363,7 +1364,7 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
                 const int64_t p0 = s->patch_begin[i], pn = H[i].n;
                 bbrt::finish_host_scene(H[i], std::vector<double>(ms.begin() + p0, ms.begin() + p0 + pn),
                                         std::vector<int32_t>(fin.begin() + p0, fin.begin() + p0 + pn),
-                                        std::vector<double>(hm.begin() + p0, hm.begin() + p0 + pn));
+                                        std::vector<double>(hm.begin() + p0, hm.begin() + p0 + pn), false);
             }, who);
             merge_meta();
             check(hipMemcpy(d_meta, meta.data(), meta.size() * sizeof(PatchMeta), hipMemcpyHostToDevice), "H2D meta");
@@ -1371,39 +1372,79 @@ int create_scenes(const bbrt_patch_set* sets, int32_t n_sets, bbrt_scene* out, c
         // BVHs, leaves and the sub-scene table
         std::vector<BvhNode> bvh;
         std::vector<int32_t> leaf;
-        for (int32_t i = 0; i < n_sets; ++i) {
-            const auto& h = H[i];
-            const int32_t bo = (int32_t)bvh.size(), lo = (int32_t)leaf.size();
-            for (BvhNode nd : h.bvh) {
-                if (nd.right >= 0) nd.right += bo; else nd.start += lo;
-                bvh.push_back(nd);
+        if (gpu_build) {
+            // every sub-scene's BVH built on the device from the inflated boxes and centroids
+            std::vector<double> gbox, gcent, widen;
+            gbox.reserve((size_t)n_patch * 6);
+            gcent.reserve((size_t)n_patch * 3);
+            for (const auto& h : H) {
+                gbox.insert(gbox.end(), h.pbox.begin(), h.pbox.end());
+                gcent.insert(gcent.end(), h.pcent.begin(), h.pcent.end());
+                widen.push_back(h.cull_widen);
             }
-            for (int32_t o : h.leaf) leaf.push_back(o + (int32_t)s->patch_begin[i]);
-            bbrt::SceneInfo si{};
-            si.root = bo; si.patch_begin = (int32_t)s->patch_begin[i]; si.diag = h.diag; si.cull_limit = 8.0 * h.scale;
-            si.cull_widen = h.cull_widen;
-            s->scenes.push_back(si);
-        }
-        if ((int64_t)bvh.size() >= (int64_t)1 << 28) throw std::runtime_error(std::string(who) + ": too many BVH nodes");
-        for (const BvhNode& nd : bvh)
-            if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
-                throw std::runtime_error(std::string(who) + ": BVH exceeds the packed node limits");
-        D.bvh = dev_upload(bvh, s->allocs, s->bytes);
-        {
-            std::vector<int32_t> roots;
-            std::vector<double> widen;
-            for (const auto& si : s->scenes) { roots.push_back(si.root); widen.push_back(si.cull_widen); }
-            std::vector<bbrt::Bvh4Node> bvh4;
-            bbrt::build_bvh4(bvh, roots, widen, bvh4);
-            D.bvh4 = dev_upload(bvh4, s->allocs, s->bytes);
-        }
-        D.leaf = dev_upload(leaf, s->allocs, s->bytes);
-        {
-            std::vector<double> lb(6 * std::max<size_t>(1, leaf.size()), 0.0);
+            std::vector<void*> tmp;
+            int64_t tmp_bytes = 0;
+            bbrt::DeviceTree T;
+            try {
+                const double* d_gbox = dev_upload(gbox, tmp, tmp_bytes);
+                const double* d_gcent = dev_upload(gcent, tmp, tmp_bytes);
+                bbrt::build_bvh_device(d_gbox, d_gcent, s->patch_begin, 4, widen.data(), s->stream, s->allocs, s->bytes,
+                                       T);
+            } catch (...) {
+                for (void* p : tmp) (void)hipFree(p);
+                throw;
+            }
+            for (void* p : tmp) (void)hipFree(p);
+            D.bvh = T.bvh;
+            D.bvh4 = T.bvh4;
+            D.leaf = T.leaf;
+            D.leaf_box = T.leaf_box;
+            bvh.resize((size_t)T.n_nodes);
+            leaf.resize((size_t)n_patch);
+            check(hipMemcpy(bvh.data(), T.bvh, bvh.size() * sizeof(BvhNode), hipMemcpyDeviceToHost), "D2H bvh");
+            check(hipMemcpy(leaf.data(), T.leaf, leaf.size() * sizeof(int32_t), hipMemcpyDeviceToHost), "D2H leaf");
+            for (int32_t i = 0; i < n_sets; ++i) {
+                bbrt::SceneInfo si{};
+                si.root = T.roots[i]; si.patch_begin = (int32_t)s->patch_begin[i]; si.diag = H[i].diag;
+                si.cull_limit = 8.0 * H[i].scale; si.cull_widen = H[i].cull_widen;
+                s->scenes.push_back(si);
+            }
+            if ((int64_t)bvh.size() >= (int64_t)1 << 28) throw std::runtime_error(std::string(who) + ": too many BVH nodes");
+        } else {
+            for (int32_t i = 0; i < n_sets; ++i) {
+                const auto& h = H[i];
+                const int32_t bo = (int32_t)bvh.size(), lo = (int32_t)leaf.size();
+                for (BvhNode nd : h.bvh) {
+                    if (nd.right >= 0) nd.right += bo; else nd.start += lo;
+                    bvh.push_back(nd);
+                }
+                for (int32_t o : h.leaf) leaf.push_back(o + (int32_t)s->patch_begin[i]);
+                bbrt::SceneInfo si{};
+                si.root = bo; si.patch_begin = (int32_t)s->patch_begin[i]; si.diag = h.diag; si.cull_limit = 8.0 * h.scale;
+                si.cull_widen = h.cull_widen;
+                s->scenes.push_back(si);
+            }
+            if ((int64_t)bvh.size() >= (int64_t)1 << 28) throw std::runtime_error(std::string(who) + ": too many BVH nodes");
             for (const BvhNode& nd : bvh)
-                if (nd.right < 0)
-                    for (int a = 0; a < 3; ++a) { lb[6 * (size_t)nd.start + a] = nd.lo[a]; lb[6 * (size_t)nd.start + 3 + a] = nd.hi[a]; }
-            D.leaf_box = dev_upload(lb, s->allocs, s->bytes);
+                if (nd.right >= (1 << 28) || nd.start >= (1 << 27) || nd.count > 7)
+                    throw std::runtime_error(std::string(who) + ": BVH exceeds the packed node limits");
+            D.bvh = dev_upload(bvh, s->allocs, s->bytes);
+            {
+                std::vector<int32_t> roots;
+                std::vector<double> widen;
+                for (const auto& si : s->scenes) { roots.push_back(si.root); widen.push_back(si.cull_widen); }
+                std::vector<bbrt::Bvh4Node> bvh4;
+                bbrt::build_bvh4(bvh, roots, widen, bvh4);
+                D.bvh4 = dev_upload(bvh4, s->allocs, s->bytes);
+            }
+            D.leaf = dev_upload(leaf, s->allocs, s->bytes);
+            {
+                std::vector<double> lb(6 * std::max<size_t>(1, leaf.size()), 0.0);
+                for (const BvhNode& nd : bvh)
+                    if (nd.right < 0)
+                        for (int a = 0; a < 3; ++a) { lb[6 * (size_t)nd.start + a] = nd.lo[a]; lb[6 * (size_t)nd.start + 3 + a] = nd.hi[a]; }
+                D.leaf_box = dev_upload(lb, s->allocs, s->bytes);
+            }
         }
         s->d_scenes = const_cast<bbrt::SceneInfo*>(dev_upload(s->scenes, s->allocs, s->bytes));
         D.scenes = s->d_scenes;

@@ -508,7 +508,7 @@ void scene_constants(HostScene& S, const std::vector<double>& patch_hmax) {
 }  // namespace
 
 void finish_host_scene(HostScene& S, const std::vector<double>& max_sag, const std::vector<int32_t>& finite,
-                       const std::vector<double>& hmax) {
+                       const std::vector<double>& hmax, bool build_tree) {
     for (int64_t k = 0; k < S.n; ++k) {
         const double ms = max_sag[k];
         double* b = &S.pbox[6 * k];
@@ -518,7 +518,7 @@ void finish_host_scene(HostScene& S, const std::vector<double>& max_sag, const s
         S.meta[k].cull = finite[k] ? 1 : 0;
     }
     scene_constants(S, hmax);
-    build_bvh(S.pbox, S.pcent, 4, S.bvh, S.leaf);
+    if (build_tree) build_bvh(S.pbox, S.pcent, 4, S.bvh, S.leaf);
 }
 
 void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads, bool compact, bool deferred) {

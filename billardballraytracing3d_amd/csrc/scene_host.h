@@ -147,11 +147,12 @@ BBRT_HOST_DEVICE inline void quantize_cell_box(const float* t, const float* b, u
 // deferred (compact scenes only): stop after the patch records, node-set
 // tables, centroids and uninflated node boxes; the device computes each
 // patch's max sagitta, finiteness and longest cell diagonal, then
-// finish_host_scene completes the host side and the device builds the pyramid.
+// finish_host_scene completes the host side (with the BVH unless build_tree is
+// false: the device then builds it) and the device builds the pyramid.
 void build_host_scene(const bbrt_patch_set& ps, HostScene& out, int threads, bool compact = false,
                       bool deferred = false);
 void finish_host_scene(HostScene& S, const std::vector<double>& max_sag, const std::vector<int32_t>& finite,
-                       const std::vector<double>& hmax);
+                       const std::vector<double>& hmax, bool build_tree = true);
 
 // The 4-wide nodes of every binary node at even depth below each root (roots[i],
 // its FP32 widening widen[i]); out has one entry per binary node.

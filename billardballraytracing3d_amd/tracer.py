@@ -46,16 +46,18 @@ def prepare_patches(sc, coarse: bool = False):
     return (box, cent, ms, cx, cell) if coarse else (box, cent, ms)
 
 
-def build_bvh(box, cent, leaf_threshold: int = 4):
+def build_bvh(box, cent, leaf_threshold: int = 4, gpu: bool = False):
+    """BVH(objects, leaf_threshold) over boxes [n][6] and centroids [n][3]: built on the host
+    (bbrt_build_bvh) or on the GPU (bbrt_build_bvh_gpu, the same tree)."""
     box = np.ascontiguousarray(box, dtype=np.float64); cent = np.ascontiguousarray(cent, dtype=np.float64)
     n = cent.shape[0]
     nb = np.zeros((2 * n, 6)); r = np.zeros(2 * n, np.int32); s = np.zeros(2 * n, np.int32)
     c = np.zeros(2 * n, np.int32); a = np.zeros(2 * n, np.uint8); lo = np.zeros(n, np.int32)
-    nn = L.lib().bbrt_build_bvh(L.ptr(box, L._d), L.ptr(cent, L._d), C.c_int64(n), C.c_int32(leaf_threshold),
-                                L.ptr(nb, L._d), L.ptr(r, L._i32), L.ptr(s, L._i32), L.ptr(c, L._i32),
-                                L.ptr(a, L._u8), L.ptr(lo, L._i32))
+    fn = L.lib().bbrt_build_bvh_gpu if gpu else L.lib().bbrt_build_bvh
+    nn = fn(L.ptr(box, L._d), L.ptr(cent, L._d), C.c_int64(n), C.c_int32(leaf_threshold), L.ptr(nb, L._d),
+            L.ptr(r, L._i32), L.ptr(s, L._i32), L.ptr(c, L._i32), L.ptr(a, L._u8), L.ptr(lo, L._i32))
     if nn < 0:
-        raise L.BBRTError("bbrt_build_bvh: " + L.last_error())
+        raise L.BBRTError(("bbrt_build_bvh_gpu: " if gpu else "bbrt_build_bvh: ") + L.last_error())
     return dict(box=nb[:nn], right=r[:nn], start=s[:nn], count=c[:nn], axis=a[:nn], leaf=lo)
 
 

@@ -215,8 +215,9 @@ int bbrt_scene_create_batch(const bbrt_patch_set* sets, int32_t n_sets, bbrt_sce
  * order-4 patch stays resident.  bbrt_scene_coarse is unavailable on it. */
 enum { BBRT_SCENE_COMPACT = 1u,
        BBRT_SCENE_DEVICE_BUILD = 2u };   /* with COMPACT: the per-cell work of the build (coarse grid, cell
-                                           padding, culling boxes) runs on the GPU, bit-identical to the host
-                                           build; the host keeps patch records, centroids, node boxes and BVHs */
+                                           padding, culling boxes) and every sub-scene's BVH (with its 4-wide
+                                           traversal nodes) are built on the GPU, bit-identical to the host
+                                           build; the host keeps patch records, centroids and node boxes */
 int bbrt_scene_create_ex(const bbrt_patch_set* sets, int32_t n_sets, uint32_t scene_flags, bbrt_scene* out);
 /* Sub-scene count, patch_begin [n+1], BVH root node [n], diagonal [n]; any pointer may be NULL. */
 int bbrt_scene_batch_info(bbrt_scene scene, int32_t* n_scenes, int64_t* patch_begin, int32_t* bvh_root,
@@ -306,6 +307,14 @@ int bbrt_prepare_patches(const bbrt_patch_set* patches, double* box, double* cen
 int64_t bbrt_build_bvh(const double* box, const double* centroid, int64_t n, int32_t leaf_threshold,
                        double* node_box, int32_t* right, int32_t* start, int32_t* count, uint8_t* axis,
                        int32_t* leaf_objects);
+
+/* bbrt_build_bvh with the build on the GPU (host arrays in and out): the same
+ * tree, bit for bit -- nodes, boxes, split axes, child links and the leaf
+ * order that libstdc++'s std::nth_element leaves (BVH.hpp:166-179).  The
+ * device build of scene handles (BBRT_SCENE_DEVICE_BUILD) runs the same code. */
+int64_t bbrt_build_bvh_gpu(const double* box, const double* centroid, int64_t n, int32_t leaf_threshold,
+                           double* node_box, int32_t* right, int32_t* start, int32_t* count, uint8_t* axis,
+                           int32_t* leaf_objects);
 
 /* BounceMapDriver::run (include/BounceMapDriver.hpp:103-259) on the device for
  * n_sample patches of a one-scene handle (patches [n_sample], host array of
