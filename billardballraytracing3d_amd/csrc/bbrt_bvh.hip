@@ -14,9 +14,10 @@
 //     before any key is compared.
 // The host lays out those nodes (BvhTask) depth by depth; one workgroup per
 // node then folds the boxes and centroid bounds, picks the axis, and runs the
-// library's nth_element order on the node's range (bvh_select.h): in LDS when
-// the range fits, else on device memory by one lane until the live range fits,
-// then in LDS.  The folds reproduce the reference's sequential min/max exactly
+// library's nth_element order on the node's range (bvh_select.h): in LDS by one
+// lane when the range fits, else round by round on device memory with the
+// whole workgroup (each partition round computed from prefix-summed ranks)
+// until the live range fits, then in LDS.  The folds reproduce the reference's sequential min/max exactly
 // (first occurrence among equal values, a NaN start kept), so boxes match bit
 // for bit, signed zeros included.
 #include <hip/hip_runtime.h>
@@ -64,14 +65,26 @@ __device__ __forceinline__ void fold_hi(double& v, int32_t& i, double v2, int32_
     if (take) { v = v2; i = i2; }
 }
 
+constexpr int kMaxWaves = 16;   // workgroups of up to 1024 threads
+
+struct BlockShared {
+    double v[kMaxWaves][6];
+    int32_t i[kMaxWaves][6];
+    int32_t part[2][kMaxWaves];
+    int32_t bcast[4];
+    SelectState st;
+    int32_t heap_done;
+};
+
 // Folds three lo and three hi coordinates of `vals` (stride `stride`, lo at +0..2, hi at +hi_off..)
-// over the objects perm[begin .. begin+n) across the wave.
+// over the objects perm[begin .. begin+n) across the workgroup; every thread gets the result.
 __device__ __forceinline__ void fold_bounds(const double* __restrict__ vals, int stride, int hi_off,
-                                            const int32_t* __restrict__ perm, int32_t begin, int32_t n, int lane,
-                                            double lo[3], double hi[3]) {
+                                            const int32_t* __restrict__ perm, int32_t begin, int32_t n,
+                                            BlockShared& sh, double lo[3], double hi[3]) {
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x, lane = tid & 63, w = tid >> 6, nw = nt >> 6;
     int32_t ilo[3], ihi[3];
     for (int a = 0; a < 3; ++a) { lo[a] = hi[a] = __builtin_nan(""); ilo[a] = ihi[a] = INT_MAX; }
-    for (int32_t j = lane; j < n; j += 64) {
+    for (int32_t j = tid; j < n; j += nt) {
         const double* x = vals + (size_t)stride * perm[begin + j];
         for (int a = 0; a < 3; ++a) { fold_lo(lo[a], ilo[a], x[a], j); fold_hi(hi[a], ihi[a], x[hi_off + a], j); }
     }
@@ -82,6 +95,19 @@ __device__ __forceinline__ void fold_bounds(const double* __restrict__ vals, int
             fold_lo(lo[a], ilo[a], l2, il);
             fold_hi(hi[a], ihi[a], h2, ih);
         }
+    if (nw > 1) {
+        if (lane == 0)
+            for (int a = 0; a < 3; ++a) {
+                sh.v[w][a] = lo[a]; sh.i[w][a] = ilo[a]; sh.v[w][3 + a] = hi[a]; sh.i[w][3 + a] = ihi[a];
+            }
+        __syncthreads();
+        for (int u = 0; u < nw; ++u)
+            for (int a = 0; a < 3; ++a) {
+                fold_lo(lo[a], ilo[a], sh.v[u][a], sh.i[u][a]);
+                fold_hi(hi[a], ihi[a], sh.v[u][3 + a], sh.i[u][3 + a]);
+            }
+        __syncthreads();
+    }
     const double* x0 = vals + (size_t)stride * perm[begin];
     for (int a = 0; a < 3; ++a) {
         if (x0[a] != x0[a]) lo[a] = x0[a];
@@ -89,27 +115,120 @@ __device__ __forceinline__ void fold_bounds(const double* __restrict__ vals, int
     }
 }
 
+// Exclusive prefix sums of two counts across the workgroup; tot gets their totals.
+__device__ __forceinline__ void block_scan2(int32_t c0, int32_t c1, BlockShared& sh, int32_t& e0, int32_t& e1,
+                                            int32_t& t0, int32_t& t1) {
+    const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6, nw = (int)blockDim.x >> 6;
+    int32_t x0 = c0, x1 = c1;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y0 = __shfl_up(x0, off), y1 = __shfl_up(x1, off);
+        if (lane >= off) { x0 += y0; x1 += y1; }
+    }
+    if (lane == 63) { sh.part[0][w] = x0; sh.part[1][w] = x1; }
+    __syncthreads();
+    int32_t b0 = 0, b1 = 0;
+    t0 = 0; t1 = 0;
+    for (int u = 0; u < nw; ++u) {
+        if (u < w) { b0 += sh.part[0][u]; b1 += sh.part[1][u]; }
+        t0 += sh.part[0][u]; t1 += sh.part[1][u];
+    }
+    __syncthreads();
+    e0 = b0 + x0 - c0;
+    e1 = b1 + x1 - c1;
+}
+
+// One partition round of introselect over a[lo+1 .. hi) around the pivot a[lo], by the whole workgroup,
+// in the rank form of bvh_select.h's partition_by_ranks: the left scan's stoppers (not below the pivot) go to
+// lpos by rank from the left, the right scan's (not above it, the pivot included) to rpos by rank from the
+// right, the first K pairs with lpos[k] < rpos[k] are swapped, and the round ends at min(lpos[K], rpos[K-1]).
+// Every thread gets the cut; the array is complete on return.
+__device__ int32_t partition_round(KeyId* __restrict__ a, int32_t lo, int32_t hi, int32_t* __restrict__ lpos,
+                                   int32_t* __restrict__ rpos, BlockShared& sh) {
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
+    const double P = a[lo].key;
+    int32_t cl = 0, cr = 0;
+    for (int32_t p = lo + tid; p < hi; p += nt) {
+        const double x = a[p].key;
+        cl += (p > lo && !(x < P)) ? 1 : 0;
+        cr += !(P < x) ? 1 : 0;
+    }
+    int32_t d0, d1, nl, nr;
+    block_scan2(cl, cr, sh, d0, d1, nl, nr);
+    int32_t carry_l = 0, carry_r = 0, kcount = 0;
+    constexpr int kItems = 8;
+    for (int32_t cb = lo; cb < hi; cb += nt * kItems) {
+        const int32_t p0 = cb + tid * kItems;
+        uint32_t fl = 0, fr = 0;
+        for (int it = 0; it < kItems; ++it) {
+            const int32_t p = p0 + it;
+            if (p < hi) {
+                const double x = a[p].key;
+                if (p > lo && !(x < P)) fl |= 1u << it;
+                if (!(P < x)) fr |= 1u << it;
+            }
+        }
+        int32_t el, er, tl, tr;
+        block_scan2(__popc(fl), __popc(fr), sh, el, er, tl, tr);
+        int32_t rank_l = carry_l + el, r_incl = carry_r + er;
+        for (int it = 0; it < kItems; ++it) {
+            const int32_t p = p0 + it;
+            if ((fr >> it) & 1u) {
+                ++r_incl;
+                rpos[nr - r_incl] = p;
+            }
+            if ((fl >> it) & 1u) {
+                lpos[rank_l] = p;
+                if (nr - r_incl > rank_l) ++kcount;
+                ++rank_l;
+            }
+        }
+        carry_l += tl;
+        carry_r += tr;
+    }
+    int32_t dk, dz, K, zz;
+    block_scan2(kcount, 0, sh, dk, dz, K, zz);   // also orders the lpos / rpos stores before the swaps
+    for (int32_t k = tid; k < K; k += nt) {
+        const int32_t x = lpos[k], y = rpos[k];
+        const KeyId t = a[x];
+        a[x] = a[y];
+        a[y] = t;
+    }
+    if (tid == 0) {
+        int32_t cut;
+        if (K == 0) cut = nl > 0 ? lpos[0] : hi - 1;
+        else cut = (K < nl && lpos[K] < rpos[K - 1]) ? lpos[K] : rpos[K - 1];
+        sh.bcast[0] = cut;
+    }
+    __syncthreads();
+    const int32_t cut = sh.bcast[0];
+    __syncthreads();
+    return cut;
+}
+
 __global__ void k_iota(int32_t* __restrict__ perm, int64_t n) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) perm[g] = (int32_t)g;
 }
 
-// One node per 64-thread workgroup (BVH.hpp:136-189): box, then for an internal
-// node the centroid bounds, split axis and the nth_element order of its range.
-__global__ void __launch_bounds__(64) k_bvh_level(const BvhTask* __restrict__ tasks, const double* __restrict__ box,
-                                                 const double* __restrict__ cent, int32_t* __restrict__ perm,
-                                                 KeyId* __restrict__ work, BvhNode* __restrict__ nodes,
-                                                 double* __restrict__ leaf_box, int lds_cap) {
+// One node per workgroup (BVH.hpp:136-189): box, then for an internal node the
+// centroid bounds, split axis and the nth_element order of its range.  A range
+// longer than lds_cap runs its partition rounds on device memory with the
+// whole workgroup (partition_round) until the live range fits in LDS, where one
+// lane finishes the select.
+__global__ void __launch_bounds__(1024) k_bvh_level(const BvhTask* __restrict__ tasks, const double* __restrict__ box,
+                                                   const double* __restrict__ cent, int32_t* __restrict__ perm,
+                                                   KeyId* __restrict__ work, int32_t* __restrict__ scr_l,
+                                                   int32_t* __restrict__ scr_r, BvhNode* __restrict__ nodes,
+                                                   double* __restrict__ leaf_box, int lds_cap) {
     extern __shared__ KeyId lds[];
-    __shared__ SelectState sh_st;
-    __shared__ int sh_done;
+    __shared__ BlockShared sh;
     const BvhTask T = tasks[blockIdx.x];
-    const int lane = (int)threadIdx.x;
+    const int tid = (int)threadIdx.x, nt = (int)blockDim.x;
 
     double lo[3], hi[3];
-    fold_bounds(box, 6, 3, perm, T.begin, T.n, lane, lo, hi);   // BVH.hpp:141-143
-    if (T.right < 0) {                                          // leaf, BVH.hpp:145-152
-        if (lane == 0) {
+    fold_bounds(box, 6, 3, perm, T.begin, T.n, sh, lo, hi);   // BVH.hpp:141-143
+    if (T.right < 0) {                                        // leaf, BVH.hpp:145-152
+        if (tid == 0) {
             BvhNode nd;
             for (int a = 0; a < 3; ++a) { nd.lo[a] = lo[a]; nd.hi[a] = hi[a]; }
             nd.right = -1; nd.start = T.begin; nd.count = T.n; nd.axis = 0;
@@ -120,50 +239,61 @@ __global__ void __launch_bounds__(64) k_bvh_level(const BvhTask* __restrict__ ta
         return;
     }
     double cmin[3], cmax[3];
-    fold_bounds(cent, 3, 0, perm, T.begin, T.n, lane, cmin, cmax);   // BVH.hpp:155-161
+    fold_bounds(cent, 3, 0, perm, T.begin, T.n, sh, cmin, cmax);   // BVH.hpp:155-161
     const double spread[3] = { cmax[0] - cmin[0], cmax[1] - cmin[1], cmax[2] - cmin[2] };
-    int axis = 0;                                                    // std::max_element, BVH.hpp:162-164
+    int axis = 0;                                                  // std::max_element, BVH.hpp:162-164
     if (spread[axis] < spread[1]) axis = 1;
     if (spread[axis] < spread[2]) axis = 2;
 
-    const int64_t b0 = T.begin, e0 = b0 + T.n, nth = b0 + T.n / 2;   // BVH.hpp:166-175
+    const int32_t b0 = T.begin, e0 = b0 + T.n, nth = b0 + T.n / 2;   // BVH.hpp:166-175
     if (T.n <= lds_cap) {
-        for (int32_t j = lane; j < T.n; j += 64) {
+        for (int32_t j = tid; j < T.n; j += nt) {
             const int32_t id = perm[b0 + j];
             lds[j] = KeyId{ cent[3 * (size_t)id + axis], id, 0 };
         }
         __syncthreads();
-        if (lane == 0) bbrt::sel::nth_element(lds, (int64_t)0, nth - b0, (int64_t)T.n, KeyLess());
+        if (tid == 0) bbrt::sel::nth_element(lds, (int64_t)0, (int64_t)(nth - b0), (int64_t)T.n, KeyLess());
         __syncthreads();
-        for (int32_t j = lane; j < T.n; j += 64) perm[b0 + j] = lds[j].id;
+        for (int32_t j = tid; j < T.n; j += nt) perm[b0 + j] = lds[j].id;
     } else {
-        for (int32_t j = lane; j < T.n; j += 64) {
+        for (int32_t j = tid; j < T.n; j += nt) {
             const int32_t id = perm[b0 + j];
             work[b0 + j] = KeyId{ cent[3 * (size_t)id + axis], id, 0 };
         }
         __syncthreads();
-        if (lane == 0) {
-            SelectState st = bbrt::sel::select_begin(b0, nth, e0);
-            sh_done = bbrt::sel::select_run(work, st, (int64_t)lds_cap, KeyLess()) ? 1 : 0;
-            sh_st = st;
+        // introselect's rounds (bvh_select.h select_run) while the live range is long
+        SelectState st = bbrt::sel::select_begin(b0, nth, e0);
+        bool heap_done = false;
+        while (st.hi - st.lo >= lds_cap) {
+            if (st.depth == 0) {   // the heap_select fallback, by one lane
+                if (tid == 0) bbrt::sel::select_run(work, st, (int64_t)0, KeyLess());
+                heap_done = true;
+                break;
+            }
+            --st.depth;
+            if (tid == 0)
+                bbrt::sel::median_to(work, st.lo, st.lo + 1, st.lo + (st.hi - st.lo) / 2, st.hi - 1, KeyLess());
+            __syncthreads();
+            const int32_t cut = partition_round(work, (int32_t)st.lo, (int32_t)st.hi, scr_l + b0, scr_r + b0, sh);
+            if (cut <= st.nth) st.lo = cut;
+            else st.hi = cut;
         }
         __syncthreads();
-        if (!sh_done) {   // the live range is now shorter than lds_cap: finish it in LDS
-            const SelectState st = sh_st;
+        if (!heap_done) {   // the live range fits in LDS: one lane finishes the select there
             const int64_t m = st.hi - st.lo;
-            for (int64_t j = lane; j < m; j += 64) lds[j] = work[st.lo + j];
+            for (int64_t j = tid; j < m; j += nt) lds[j] = work[st.lo + j];
             __syncthreads();
-            if (lane == 0) {
+            if (tid == 0) {
                 SelectState s2{ 0, m, st.nth - st.lo, st.depth };
                 bbrt::sel::select_run(lds, s2, (int64_t)0, KeyLess());
             }
             __syncthreads();
-            for (int64_t j = lane; j < m; j += 64) work[st.lo + j] = lds[j];
+            for (int64_t j = tid; j < m; j += nt) work[st.lo + j] = lds[j];
             __syncthreads();
         }
-        for (int32_t j = lane; j < T.n; j += 64) perm[b0 + j] = work[b0 + j].id;
+        for (int32_t j = tid; j < T.n; j += nt) perm[b0 + j] = work[b0 + j].id;
     }
-    if (lane == 0) {   // BVH.hpp:183-187
+    if (tid == 0) {   // BVH.hpp:183-187
         BvhNode nd;
         for (int a = 0; a < 3; ++a) { nd.lo[a] = lo[a]; nd.hi[a] = hi[a]; }
         nd.right = T.right; nd.start = 0; nd.count = 0; nd.axis = axis;
@@ -288,6 +418,8 @@ void build_bvh_device(const double* d_box, const double* d_cent, const std::vect
     try {
         BvhTask* d_tasks = dev_alloc<BvhTask>(tasks.size(), tmp, tmp_bytes);
         KeyId* d_work = big ? dev_alloc<KeyId>((size_t)n_obj, tmp, tmp_bytes) : nullptr;
+        int32_t* d_scr_l = big ? dev_alloc<int32_t>((size_t)n_obj, tmp, tmp_bytes) : nullptr;
+        int32_t* d_scr_r = big ? dev_alloc<int32_t>((size_t)n_obj, tmp, tmp_bytes) : nullptr;
         check(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(BvhTask), hipMemcpyHostToDevice, st), "H2D");
         out.bvh = dev_alloc<BvhNode>((size_t)out.n_nodes, allocs, bytes);
         out.leaf = dev_alloc<int32_t>((size_t)n_obj, allocs, bytes);
@@ -299,8 +431,11 @@ void build_bvh_device(const double* d_box, const double* d_cent, const std::vect
         for (size_t d = 0; d + 1 < level_off.size(); ++d) {
             const int64_t cnt = level_off[d + 1] - level_off[d];
             const int cap = std::min(kLdsCap, std::max(4, level_max[d]));
-            hipLaunchKernelGGL(k_bvh_level, dim3((unsigned)cnt), dim3(64), (size_t)cap * sizeof(KeyId), st,
-                               d_tasks + level_off[d], d_box, d_cent, out.leaf, d_work, out.bvh, out.leaf_box, cap);
+            // long ranges run their partition rounds with 256 threads, the rest with one wave
+            const int threads = level_max[d] > kLdsCap ? 256 : 64;
+            hipLaunchKernelGGL(k_bvh_level, dim3((unsigned)cnt), dim3(threads), (size_t)cap * sizeof(KeyId), st,
+                               d_tasks + level_off[d], d_box, d_cent, out.leaf, d_work, d_scr_l, d_scr_r, out.bvh,
+                               out.leaf_box, cap);
             check(hipGetLastError(), "k_bvh_level");
         }
         if (widen) {

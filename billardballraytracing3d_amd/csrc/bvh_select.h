@@ -82,6 +82,31 @@ __host__ __device__ inline int64_t partition_around(E* a, int64_t lo, int64_t hi
     }
 }
 
+// The same partition round from ranks, the form the device runs in parallel.
+// Let L be the positions in [lo, hi) whose element is not below the pivot
+// (the left scan's stoppers), ascending, and R the positions in [pv, hi) whose
+// element is not above it (the right scan's stoppers, the pivot itself
+// included), descending.  Between swaps each scan sees original elements
+// until it reaches a place the other scan has filled, which holds a stopper
+// for it; so the k-th swap exchanges L[k] and R[k], swaps happen exactly for
+// the prefix of k with L[k] < R[k] (L rises, R falls), say k < K, and the left
+// scan then stops at min(L[K], R[K-1]) -- at L[0] when K = 0.  lpos and rpos
+// receive L and R (room for hi - pv entries each).
+template <class E, class Less>
+__host__ __device__ inline int64_t partition_by_ranks(E* a, int64_t lo, int64_t hi, int64_t pv, Less lt,
+                                                      int64_t* lpos, int64_t* rpos) {
+    int64_t nl = 0, nr = 0;
+    for (int64_t p = lo; p < hi; ++p)
+        if (!lt(a[p], a[pv])) lpos[nl++] = p;
+    for (int64_t p = hi - 1; p >= pv; --p)
+        if (!lt(a[pv], a[p])) rpos[nr++] = p;
+    int64_t K = 0;
+    while (K < nl && K < nr && lpos[K] < rpos[K]) ++K;
+    for (int64_t k = 0; k < K; ++k) swap_at(a, lpos[k], rpos[k]);
+    if (K == 0) return nl > 0 ? lpos[0] : hi - 1;
+    return (K < nl && lpos[K] < rpos[K - 1]) ? lpos[K] : rpos[K - 1];
+}
+
 template <class E, class Less>
 __host__ __device__ inline void insertion_sort(E* a, int64_t lo, int64_t hi, Less lt) {
     if (lo == hi) return;

@@ -1,6 +1,8 @@
 // Checks bbrt::sel::nth_element (billardballraytracing3d_amd/csrc/bvh_select.h,
 // the device BVH build's split) against std::nth_element: the whole permutation
 // must be the same, element for element, since it becomes the BVH's leaf order.
+// The rank form of a partition round (partition_by_ranks, what the device runs
+// in parallel on long ranges) must give the scans' array and cut every round.
 // Inputs: random keys (distinct, few distinct values, many equal), sorted,
 // reversed, organ-pipe, and inputs built by McIlroy's adversary ("A killer
 // adversary for quicksort", 1999) against the median-of-three partition, which
@@ -36,12 +38,37 @@ static bool same(const std::vector<KeyId>& x, const std::vector<KeyId>& y) {
     return true;
 }
 
+// introselect whose partition rounds are computed from ranks (partition_by_ranks), as the device runs the
+// rounds of long ranges; everything else is select_run's
+static void nth_element_by_ranks(std::vector<KeyId>& v, int64_t nth) {
+    KeyId* a = v.data();
+    const int64_t n = (int64_t)v.size();
+    if (n == 0 || nth == n) return;
+    std::vector<int64_t> lpos((size_t)n), rpos((size_t)n);
+    bbrt::sel::SelectState st = bbrt::sel::select_begin(0, nth, n);
+    while (st.hi - st.lo > 3 && st.depth > 0) {
+        --st.depth;
+        bbrt::sel::median_to(a, st.lo, st.lo + 1, st.lo + (st.hi - st.lo) / 2, st.hi - 1, KeyLess());
+        std::vector<KeyId> w(v);
+        const int64_t c0 = bbrt::sel::partition_around(w.data(), st.lo + 1, st.hi, st.lo, KeyLess());
+        const int64_t cut = bbrt::sel::partition_by_ranks(a, st.lo + 1, st.hi, st.lo, KeyLess(), lpos.data(), rpos.data());
+        if (cut != c0 || !same(v, w)) {
+            if (g_fail < 10) std::printf("MISMATCH partition round n=%lld\n", (long long)n);
+            ++g_fail;
+        }
+        if (cut <= st.nth) st.lo = cut;
+        else st.hi = cut;
+    }
+    bbrt::sel::select_run(a, st, 0, KeyLess());
+}
+
 static void check(const std::vector<double>& keys, int64_t nth, const char* what) {
-    std::vector<KeyId> a = with_keys(keys), b = a;
+    std::vector<KeyId> a = with_keys(keys), b = a, c = a;
     bbrt::sel::nth_element(a.data(), 0, nth, (int64_t)a.size(), KeyLess());
     std::nth_element(b.begin(), b.begin() + nth, b.end(), [](const KeyId& x, const KeyId& y) { return x.key < y.key; });
+    nth_element_by_ranks(c, nth);
     ++g_cases;
-    if (!same(a, b)) {
+    if (!same(a, b) || !same(c, b)) {
         if (g_fail < 10) std::printf("MISMATCH %s n=%zu nth=%lld\n", what, keys.size(), (long long)nth);
         ++g_fail;
     }
