@@ -4,8 +4,8 @@ Workload (default): the Sinai-like scene -- 4 spheres + 1 ellipsoid, 2,048
 order-4 (5x5 tensor) patches each, 10,240 patches -- and 2^20 rays from the
 z=-8 plane in a 10 degree cone, max 100 bounces, FP64 Newton in the
 `converged` mode, triangle (stage-1) seeds.  One step = one trace of the whole
-ray batch, inputs and outputs resident in HBM.  By default two steps are kept
-in flight on two streams (bbrt_trace_device_async), each with its own outputs:
+ray batch, inputs and outputs resident in HBM.  By default four steps are kept
+in flight on four streams (bbrt_trace_device_async), each with its own outputs:
 the next step's rays start on the CUs the previous step's last rays leave.  A
 pass with one synchronous trace at a time is timed too (`one_in_flight`) and
 gives the roofline's kernel launch times.
@@ -26,9 +26,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 # Streams that share a hardware queue run their kernels in order.  With the
-# HIP default of 4 queues per process, the first two streams created after a
-# scene handle's own streams share one, which would serialise the in-flight
-# traces; 8 queues give each of them its own.  Set before the HIP runtime starts.
+# HIP default of 4 queues per process, the first streams created after a scene
+# handle's own two share queues, which would serialise the in-flight traces;
+# 8 queues give the handle's streams and four in-flight streams one each.  Set
+# before the HIP runtime starts.
 os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 METRIC = "converged bounces/s at ≤1e-10 hit error; mean Newton iters, learned vs tri seed"
@@ -47,7 +48,7 @@ def main():
     ap.add_argument("--mode", choices=["ref", "converged"], default="converged")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle time budget (0 = skip)")
-    ap.add_argument("--in-flight", type=int, default=2,
+    ap.add_argument("--in-flight", type=int, default=4,
                     help="Sinai headline: traces kept in flight on that many streams (bbrt_trace_device_async); "
                          "1 = one synchronous trace at a time")
     ap.add_argument("--in-flight-order", choices=["grouped", "input"], default="grouped",
