@@ -700,13 +700,24 @@ def learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value_tri
     mlp_tflops = evals * MLP_FLOPS_PER_WAVE_EVAL / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
 
     def records_vs_triangle():
-        same = (outs["status"] == ref["status"]) & (outs["err_kind"] == ref["err_kind"]) & \
-            (outs["bounces"] == ref["bounces"])
+        """(rays whose records differ from the triangle-seed run's, the bounce index at which each such ray's
+        record first departs: the first differing patch among the compared slots, else the bounce at which
+        one of the two runs ended)."""
+        nb, nb_ref = outs["bounces"], ref["bounces"]
+        differ = (outs["status"] != ref["status"]) | (outs["err_kind"] != ref["err_kind"]) | (nb != nb_ref)
+        ended = torch.minimum(nb, nb_ref).to(torch.int64)
+        first = ended.clone()
         if kk > 0:
-            slot = torch.arange(kk, device=outs["bounces"].device)[None, :]
-            valid = slot < torch.clamp(ref["bounces"], max=kk)[:, None].to(slot.dtype)
-            same &= torch.all((outs["hit_patch"][:, :kk] == ref["hit_patch"]) | ~valid, dim=1)
-        return int((~same).sum().item())
+            slot = torch.arange(kk, device=nb.device)[None, :]
+            valid = slot < torch.clamp(ended, max=kk)[:, None]
+            diffp = (outs["hit_patch"][:, :kk] != ref["hit_patch"]) & valid
+            anyp = diffp.any(dim=1)
+            differ |= anyp
+            first = torch.where(anyp, torch.argmax(diffp.to(torch.int32), dim=1).to(torch.int64), ended)
+        f = first[differ].cpu().numpy()
+        stats = None if f.size == 0 else {"min": int(f.min()), "median": float(np.median(f)), "max": int(f.max()),
+                                          "within_20": int((f < 20).sum())}
+        return int(differ.sum().item()), stats
 
     # the network-first form (BBRT_FLAG_SEED_FIRST): stage 1 only where the network seed's Newton is not decisive
     first = None
@@ -726,11 +737,13 @@ def learned_leg(args, gs, to, td, n, mb, K, outs, ptrs, stream, flags, value_tri
         torch.cuda.synchronize()
         fe = time.perf_counter()
         fv = int(fctr[L.CTR["bounces"]]) / (fe - fs)
+        first_diff = records_vs_triangle()
         first = {"value": fv, "ms_per_step": (fe - fs) * 1e3 / args.steps,
                  "vs_triangle": fv / value_tri if value_tri else None, "vs_learned_stage1_first": fv / value,
                  "per_step": {k: int(fctr[L.CTR[k]]) // args.steps for k in
                               ("seed_learned", "seed_miss", "seed_fallback", "mt_tests", "newton_iters", "bounces")},
-                 "rays_differing_from_triangle_records": records_vs_triangle(), "bounces_compared": kk,
+                 "rays_differing_from_triangle_records": first_diff[0], "first_divergent_bounce": first_diff[1],
+                 "bounces_compared": kk,
                  "note": "network seed to Newton first; in-domain root = hit, converged outside the patch by more "
                          "than the margin = miss, else stage 1 + triangle seed; records measured against the "
                          "triangle-seed run (not promised equal)"}

@@ -230,6 +230,11 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         __syncthreads();
         lds = &mlp_lds;
     }
+    OwnerRay* owner_slots = nullptr;   // compact scenes: this wave's rays for the wave-wide cell tests
+    if constexpr (COMPACT) {
+        __shared__ OwnerRay owner_lds[256];
+        owner_slots = owner_lds + (threadIdx.x & ~63u);
+    }
     const int n_live = *n_live_p;
     const int lane = __lane_id();
     Counters c = {};
@@ -460,7 +465,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                         compact_walk_init(S, obj, r32, cull && S.meta[obj].cull, w, c);
                     }
                 }
-                compact_cells_wave(S, tile, obj, ti, tj, mc, o, d, r32, widen, w.use_cull, s1, c);
+                compact_cells_wave(S, tile, obj, ti, tj, mc, o, d, r32, widen, w.use_cull, s1, c, owner_slots);
             }
         } else if ((int)__popcll(in_leaf) > coop_max) {
             if (r >= 0 && phase == kPhLeaf) {

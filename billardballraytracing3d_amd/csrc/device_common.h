@@ -634,6 +634,16 @@ __device__ __forceinline__ int nth_bit(unsigned long long m, int k) {
     return lo;
 }
 
+// A lane's ray as the other lanes of its wave read it during the wave-wide
+// cell tests (one LDS slot per thread of the block).
+struct OwnerRay {
+    double ox, oy, oz, dx, dy, dz;
+    double widen;
+    float r32[6];
+    int32_t use_cull;
+    int32_t pad;
+};
+
 // The cells the lanes' walks selected, tested by the whole wave (every lane
 // calls this; `have` marks the lanes with a tile).  The wave's list holds the
 // lanes' cells in lane order, each lane's in bit order; lane e of a pass takes
@@ -641,10 +651,13 @@ __device__ __forceinline__ int nth_bit(unsigned long long m, int k) {
 // the owners fold the entries' results into their stage-1 best.  The cells
 // tested and the counts are the per-lane walk's, and the best is the minimum
 // of (t, key), so the result is the per-lane walk's too; but a cell costs one
-// lane of a 64-wide pass instead of a pass of its own.
+// lane of a 64-wide pass instead of a pass of its own.  The owners' rays go
+// through LDS (`slots`, this wave's 64 entries) and are read only once the
+// cell is evaluated, so they hold no registers during its field loads.
 __device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have, int32_t k, int ti, int tj,
                                                    uint32_t mc, const V3& o, const V3& d, const Ray32& r,
-                                                   double widen, bool use_cull, Stage1& best, Counters& c) {
+                                                   double widen, bool use_cull, Stage1& best, Counters& c,
+                                                   OwnerRay* slots) {
     const int lane = __lane_id();
     const int cnt = have ? __popc(mc) : 0;
     int incl = cnt;
@@ -654,7 +667,19 @@ __device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have,
         if (lane >= off) incl += v;
     }
     const int total = __shfl(incl, 63);
+    if (total == 0) return;
     const int start = incl - cnt;
+    if (have) {
+        OwnerRay& me = slots[lane];
+        me.ox = o.x; me.oy = o.y; me.oz = o.z; me.dx = d.x; me.dy = d.y; me.dz = d.z;
+        me.widen = widen;
+        me.r32[0] = r.ox; me.r32[1] = r.oy; me.r32[2] = r.oz; me.r32[3] = r.ix; me.r32[4] = r.iy; me.r32[5] = r.iz;
+        me.use_cull = use_cull ? 1 : 0;
+    }
+    // the wave's LDS writes land before any lane of it reads them (one wave, in-order LDS)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int p0 = 0; p0 < total; p0 += 64) {
         const int e = p0 + lane;
         int own = 0;   // lanes whose cells all come before entry e: its owner
@@ -664,16 +689,19 @@ __device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have,
         const int32_t kk = __shfl(k, own);
         const int oti = __shfl(ti, own), otj = __shfl(tj, own), ost = __shfl(start, own);
         const uint32_t omc = (uint32_t)__shfl((int)mc, own);
-        const V3 oo = { __shfl(o.x, own), __shfl(o.y, own), __shfl(o.z, own) };
-        const V3 od = { __shfl(d.x, own), __shfl(d.y, own), __shfl(d.z, own) };
-        const Ray32 orr = { __shfl(r.ox, own), __shfl(r.oy, own), __shfl(r.oz, own),
-                            __shfl(r.ix, own), __shfl(r.iy, own), __shfl(r.iz, own) };
-        const double ow = __shfl(widen, own);
-        const bool ouc = __shfl(use_cull ? 1 : 0, own) != 0;
         Stage1 ib{ 0.0, 0, false };
         if (e < total) {
             const int cq = nth_bit(omc, e - ost);
-            compact_cell_tests(S, kk, 3 * oti + cq / 3, 3 * otj + cq % 3, oo, od, orr, ow, ouc, ib, c);
+            const int ci = 3 * oti + cq / 3, cj = 3 * otj + cq % 3;
+            const CellData cd = compact_cell_any(S, kk, ci, cj);
+            // the owner's ray only now: it is not held across the cell's field loads
+            __builtin_amdgcn_sched_barrier(0);
+            const OwnerRay& R = slots[own];
+            const Ray32 orr = { R.r32[0], R.r32[1], R.r32[2], R.r32[3], R.r32[4], R.r32[5] };
+            if (!R.use_cull || compact_cell_box(cd, R.widen, orr)) {
+                const V3 oo = { R.ox, R.oy, R.oz }, od = { R.dx, R.dy, R.dz };
+                cell_tests_data(cd, ci, cj, oo, od, ib, c);
+            }
         }
         // each owner's entries of this pass: list entries start .. start + cnt - 1
         for (int u = 0; u < 9; ++u) {
@@ -689,6 +717,8 @@ __device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have,
             }
         }
     }
+    // the slots are written again by the next call: every lane's reads of this call come first
+    __builtin_amdgcn_wave_barrier();
 }
 
 struct PHit {
