@@ -508,39 +508,53 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         const unsigned long long busy = __ballot((r >= 0 && phase != kPhSeed) || r == -1);
         if (pend && ((int)__popcll(pend) >= newton_batch || busy == 0)) {
             const bool need = r >= 0 && phase == kPhSeed;
-            if constexpr (LEARNED) {
+            if constexpr (LEARNED && !FIRST) {
+                // Newton from the network's seed; lanes whose root fails the acceptance rule go round once
+                // more from the triangle seed.  One call site: the solver is inlined once.
+                double ul, vl, tl;
+                mlp_seed_wave(MD, *lds, need, obj, o, d, ul, vl, tl);
+                if (lane == 0) c.mlp_evals++;
+                if (need) c.mlp_lanes++;
+                bool again = need;
+                for (int attempt = 0; __ballot(again) != 0; ++attempt) {
+                    if (again) {
+                        h = newton_any<F32>(S, obj, o, d, ul, vl, tl, mode, c);
+                        again = false;
+                        if (attempt == 0) {
+                            const bool ok = h.status == 1 && h.polish_ok && fabs(h.u - sd.u) <= mlp::kSeedWindow &&
+                                            fabs(h.v - sd.v) <= mlp::kSeedWindow;
+                            if (ok) {
+                                c.seed_learned++;
+                            } else {
+                                c.seed_fallback++;
+                                ul = sd.u; vl = sd.v; tl = sd.t;
+                                again = true;
+                            }
+                        }
+                    }
+                }
+            } else if constexpr (LEARNED) {   // FIRST: the network seed before stage 1
                 double ul, vl, tl;
                 mlp_seed_wave(MD, *lds, need, obj, o, d, ul, vl, tl);
                 if (lane == 0) c.mlp_evals++;
                 if (need) {
                     c.mlp_lanes++;
                     h = newton_any<F32, FIRST>(S, obj, o, d, ul, vl, tl, mode, c);
-                    if constexpr (FIRST) {
-                        constexpr double lim = 1.0 + BBRT_SEED_FIRST_MISS_MARGIN;
-                        // a root behind the origin or at it (the point the ray was nudged off) is no stage-1
-                        // hit: Moller-Trumbore rejects t < 1e-9 (MollerTrumbore.hpp:54-55)
-                        if (h.status == 1 && h.polish_ok && h.t >= kMtEps) {
-                            c.seed_learned++;
-                        } else if (h.status == -bbrt::kErrDomain && (fabs(h.u) > lim || fabs(h.v) > lim)) {
-                            c.seed_miss++;
-                            h = PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
-                        } else {
-                            // stage 1 and the triangle seed decide (SurfacePatch.hpp:253-303)
-                            c.seed_fallback++;
-                            c.candidates--;   // stage1_seed counts the candidate again
-                            sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
-                            h = sd.have ? newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c)
-                                        : PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
-                        }
+                    constexpr double lim = 1.0 + BBRT_SEED_FIRST_MISS_MARGIN;
+                    // a root behind the origin or at it (the point the ray was nudged off) is no stage-1
+                    // hit: Moller-Trumbore rejects t < 1e-9 (MollerTrumbore.hpp:54-55)
+                    if (h.status == 1 && h.polish_ok && h.t >= kMtEps) {
+                        c.seed_learned++;
+                    } else if (h.status == -bbrt::kErrDomain && (fabs(h.u) > lim || fabs(h.v) > lim)) {
+                        c.seed_miss++;
+                        h = PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
                     } else {
-                        const bool ok = h.status == 1 && h.polish_ok && fabs(h.u - sd.u) <= mlp::kSeedWindow &&
-                                        fabs(h.v - sd.v) <= mlp::kSeedWindow;
-                        if (ok) {
-                            c.seed_learned++;
-                        } else {
-                            c.seed_fallback++;
-                            h = newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c);
-                        }
+                        // stage 1 and the triangle seed decide (SurfacePatch.hpp:253-303)
+                        c.seed_fallback++;
+                        c.candidates--;   // stage1_seed counts the candidate again
+                        sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
+                        h = sd.have ? newton_any<F32>(S, obj, o, d, sd.u, sd.v, sd.t, mode, c)
+                                    : PHit{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
                     }
                 }
             } else {
