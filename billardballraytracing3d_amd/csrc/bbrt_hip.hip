@@ -477,13 +477,29 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 }
             }
         } else if (!COMPACT && in_leaf) {
-            // few lanes left here: the whole wave runs each one's stage 1 in turn
+            // few lanes left here: the whole wave runs each one's stage 1 in turn.  First every leaf lane tests
+            // its object's nine super boxes itself, all in one round trip: an object none of whose supers the ray
+            // enters has no stage-1 hit (counted as the walk counts it), and the lane moves on; the wave-wide
+            // walks start at the blocks of the hit supers.  The boxes, counts and results are the walk's own.
+            uint32_t sup = 0;
+            auto super_level = [&]() {   // from the lane's current entered object: the next one the walk must see
+                for (;;) {
+                    obj = S.leaf[leaf_start + q];
+                    if (!(cull && S.meta[obj].cull)) { sup = 0; return; }   // brute force: the walk does it
+                    sup = group_mask<9, bbrt::kG9>(S.box_super + (size_t)obj * bbrt::kSuperFloats, r32);
+                    c.cull_tests += 9;
+                    if (sup) return;
+                    c.candidates++;
+                    if (!next_object()) return;
+                }
+            };
+            if (r >= 0 && phase == kPhLeaf) super_level();
             for (;;) {
                 const unsigned long long need = __ballot(r >= 0 && phase == kPhLeaf);
                 if (!need) break;
                 const int ld = __ffsll((long long)need) - 1;
-                if (lane == ld) obj = S.leaf[leaf_start + q];
                 const int32_t kk = __shfl(obj, ld);
+                const uint32_t lsup = (uint32_t)__shfl((int)sup, ld);
                 const V3 lo_ = { __shfl(o.x, ld), __shfl(o.y, ld), __shfl(o.z, ld) };
                 const V3 ld_ = { __shfl(d.x, ld), __shfl(d.y, ld), __shfl(d.z, ld) };
                 const Ray32 lr = { __shfl(r32.ox, ld), __shfl(r32.oy, ld), __shfl(r32.oz, ld),
@@ -492,13 +508,13 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 const bool lcull = __shfl(cull ? 1 : 0, ld) != 0;
                 uint32_t n_cull = 0, n_mt = 0;
                 const Stage1 b1 = stage1_coop<COMPACT ? 1 : 0>(S, kk, lo_, ld_, lr, S.scenes[lscn].cull_widen,
-                                                               lcull && S.meta[kk].cull, n_cull, n_mt);
+                                                               lcull && S.meta[kk].cull, lsup, n_cull, n_mt);
                 if (lane == ld) {
                     c.candidates++;
                     c.cull_tests += n_cull;
                     c.mt_tests += n_mt;
                     if (b1.have) { sd = seed_of(b1); phase = kPhSeed; }   // parked until the wave runs Newton
-                    else next_object();
+                    else if (next_object()) super_level();
                 }
             }
         }

@@ -1035,17 +1035,18 @@ __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const 
 
 // Stage 1 of one (ray, patch) pair by the whole wave (every lane calls it with
 // the same arguments).  The pyramid is walked level by level, each level's
-// boxes spread over the lanes (supers, then the blocks of the hit supers 4
-// supers a pass, the tiles of the hit blocks 7 a pass, the cells of the hit
-// tiles 7 a pass), and each hit cell's Moller-Trumbore tests run on a lane of
-// its own.  The cells tested are the per-lane walk's, and the best hit is the
-// minimum of (t, key) over them, so the result does not depend on the order;
-// the work counts (returned) are the per-lane walk's too.  For a wave with one
-// or two lanes left in the leaf phase, a level costs one memory round trip
-// instead of one per hit box.
+// boxes spread over the lanes (the caller's super mask ms_in, then the blocks
+// of the hit supers 4 supers a pass, the tiles of the hit blocks 7 a pass, the
+// cells of the hit tiles 7 a pass), and each hit cell's Moller-Trumbore tests
+// run on a lane of its own.  The cells tested are the per-lane walk's, and the
+// best hit is the minimum of (t, key) over them, so the result does not depend
+// on the order; the work counts (returned; the super level's 9 box tests are
+// the caller's) are the per-lane walk's too.  For a wave with one or two lanes
+// left in the leaf phase, a level costs one memory round trip instead of one
+// per hit box.
 template <int CM>
 __device__ Stage1 stage1_coop(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r, double widen,
-                              bool use_cull, uint32_t& n_cull, uint32_t& n_mt) {
+                              bool use_cull, uint32_t ms_in, uint32_t& n_cull, uint32_t& n_mt) {
     const int lane = __lane_id();
     const double* G = S.cxyz + (size_t)k * bbrt::kCoarseN * bbrt::kCoarseN * 3;
     const double* C = S.cell + (size_t)k * bbrt::kCells * bbrt::kCells * 4;
@@ -1070,13 +1071,10 @@ __device__ Stage1 stage1_coop(const DevScene& S, int64_t k, const V3& o, const V
         }
         n_mt = 6u * bbrt::kCells * bbrt::kCells;
     } else {
-        const float* gs = S.box_super + (size_t)k * bbrt::kSuperFloats;
         const float* gb = S.box_block + (size_t)k * bbrt::kBlockFloats;
         const float* gt = S.box_tile + (size_t)k * bbrt::kTileFloats;
         const float* gc = S.box_cell + (size_t)k * bbrt::kCellFloats;
-        const bool h0 = lane < 9 && box32(gs + 6 * lane, r);
-        const unsigned long long ms = __ballot(h0);
-        n_cull += 9;
+        const unsigned long long ms = ms_in;   // the super level, tested and counted by the caller
         const int ns = (int)__popcll(ms);
         for (int sb = 0; sb < ns; sb += 4) {
             // blocks of hit supers sb .. sb+3: lane = 16 * super + block slot
