@@ -438,8 +438,12 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         } else if constexpr (COMPACT) {
             // compact scenes: each leaf lane walks its object's pyramid, and the cells the walks select are
             // tested by the whole wave, 64 a pass; the walks resume until each lane's stage 1 hits (the lane
-            // parks for Newton) or its leaf ends
-            bool walking = r >= 0 && phase == kPhLeaf;
+            // parks for Newton) or its leaf ends.  Leaf lanes wait (the others traverse on) until
+            // kCompactGather of them are in a leaf or no other live lane has work before Newton, so the
+            // passes hold more cells; stage 1's outcome does not depend on when it runs.
+            const unsigned long long parked = __ballot(r >= 0 && phase == kPhSeed);
+            const bool go = (int)__popcll(in_leaf) >= kCompactGather || (in_leaf | parked) == __ballot(r >= 0);
+            bool walking = go && r >= 0 && phase == kPhLeaf;
             CompactWalk w{ 0u, 0u, 0u, 0, 0, 0, 0, 0, false };
             Stage1 s1{ 0.0, 0, false };
             const double widen = S.scenes[scn].cull_widen;
