@@ -497,8 +497,9 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                     if (!next_object()) return;
                 }
             };
-            if (r >= 0 && phase == kPhLeaf) super_level();
+            bool want_super = r >= 0 && phase == kPhLeaf;   // one call site: the lambda is inlined once
             for (;;) {
+                if (want_super) { super_level(); want_super = false; }
                 const unsigned long long need = __ballot(r >= 0 && phase == kPhLeaf);
                 if (!need) break;
                 const int ld = __ffsll((long long)need) - 1;
@@ -518,7 +519,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                     c.cull_tests += n_cull;
                     c.mt_tests += n_mt;
                     if (b1.have) { sd = seed_of(b1); phase = kPhSeed; }   // parked until the wave runs Newton
-                    else if (next_object()) super_level();
+                    else if (next_object()) want_super = true;
                 }
             }
         }
