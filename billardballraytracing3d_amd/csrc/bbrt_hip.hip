@@ -235,6 +235,11 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         __shared__ OwnerRay owner_lds[256];
         owner_slots = owner_lds + (threadIdx.x & ~63u);
     }
+    BfsOwner* bfs_slots = nullptr;     // stored scenes: this wave's owner records for stage1_bfs (dynamic LDS,
+    if constexpr (!COMPACT && !FIRST) {   // 64 records a wave)
+        extern __shared__ __align__(16) unsigned char bbrt_dyn_lds[];
+        bfs_slots = reinterpret_cast<BfsOwner*>(bbrt_dyn_lds) + (threadIdx.x & ~63u);
+    }
     const int n_live = *n_live_p;
     const int lane = __lane_id();
     Counters c = {};
@@ -426,8 +431,6 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         bool have_h = false;
         PHit h{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
         const unsigned long long in_leaf = __ballot(r >= 0 && phase == kPhLeaf);
-        const int coop_max = (flags & BBRT_FLAG_STAGE1_WAVE) ? 64
-                             : ((flags & BBRT_FLAG_STAGE1_LANE) ? 0 : (LEARNED ? kCoopLanesLearned : kCoopLanes));
         if constexpr (FIRST) {
             // straight to the network and Newton with the leaf's next entered object
             if (r >= 0 && phase == kPhLeaf) {
@@ -471,7 +474,8 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 }
                 compact_cells_wave(S, tile, obj, ti, tj, mc, o, d, r32, widen, w.use_cull, s1, c, owner_slots);
             }
-        } else if ((int)__popcll(in_leaf) > coop_max) {
+        } else if (flags & BBRT_FLAG_STAGE1_LANE) {
+            // opt-in: each lane walks its own pyramids, object after object
             if (r >= 0 && phase == kPhLeaf) {
                 for (;;) {
                     obj = S.leaf[leaf_start + q];
@@ -481,10 +485,12 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 }
             }
         } else if (!COMPACT && in_leaf) {
-            // few lanes left here: the whole wave runs each one's stage 1 in turn.  First every leaf lane tests
-            // its object's nine super boxes itself, all in one round trip: an object none of whose supers the ray
-            // enters has no stage-1 hit (counted as the walk counts it), and the lane moves on; the wave-wide
-            // walks start at the blocks of the hit supers.  The boxes, counts and results are the walk's own.
+            // Every leaf lane first tests its object's nine super boxes itself, all in one round trip: an object
+            // none of whose supers the ray enters has no stage-1 hit (counted as the walk counts it), and the lane
+            // moves on.  Then the wave walks the pyramids below the hit supers of all its leaf lanes together,
+            // breadth-first (stage1_bfs); objects without culling, and every object under
+            // BBRT_FLAG_STAGE1_WAVE, are walked by the whole wave one lane at a time (stage1_coop).  The boxes,
+            // counts and results are the walk's own.
             uint32_t sup = 0;
             auto super_level = [&]() {   // from the lane's current entered object: the next one the walk must see
                 for (;;) {
@@ -498,11 +504,27 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 }
             };
             bool want_super = r >= 0 && phase == kPhLeaf;   // one call site: the lambda is inlined once
+            const bool serial_all = (flags & BBRT_FLAG_STAGE1_WAVE) != 0;
             for (;;) {
                 if (want_super) { super_level(); want_super = false; }
-                const unsigned long long need = __ballot(r >= 0 && phase == kPhLeaf);
+                const bool mine = r >= 0 && phase == kPhLeaf;
+                const unsigned long long need = __ballot(mine);
                 if (!need) break;
-                const int ld = __ffsll((long long)need) - 1;
+                const unsigned long long serial = serial_all ? need : __ballot(mine && sup == 0);
+                if (!serial) {
+                    Stage1 b1{ 0.0, 0, false };
+                    uint32_t n_cull = 0, n_mt = 0;
+                    stage1_bfs(S, mine, obj, o, d, r32, sup, bfs_slots, b1, n_cull, n_mt);
+                    if (mine) {
+                        c.candidates++;
+                        c.cull_tests += n_cull;
+                        c.mt_tests += n_mt;
+                        if (b1.have) { sd = seed_of(b1); phase = kPhSeed; }   // parked until the wave runs Newton
+                        else if (next_object()) want_super = true;
+                    }
+                    continue;
+                }
+                const int ld = __ffsll((long long)serial) - 1;
                 const int32_t kk = __shfl(obj, ld);
                 const uint32_t lsup = (uint32_t)__shfl((int)sup, ld);
                 const V3 lo_ = { __shfl(o.x, ld), __shfl(o.y, ld), __shfl(o.z, ld) };
@@ -1103,6 +1125,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         throw std::runtime_error("bbrt_trace: BBRT_FLAG_SEED_FIRST needs the learned seed, BBRT_NEWTON_CONVERGED and "
                                  "the persistent schedule");
     unsigned grid = 1;
+    size_t lds_bytes = 0;   // dynamic LDS of the persistent kernel
     const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
     const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch;
     if (!wavefront && p->max_bounces > 0) {
@@ -1147,8 +1170,9 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             kern = wpe == 2 ? &k_persist<2, false, false, false> : (wpe == 3 ? &k_persist<3, false, false, false>
                                                                               : &k_persist<4, false, false, false>);
         }
+        lds_bytes = (!s->compact && !first) ? (size_t)block * sizeof(BfsOwner) : 0;
         int per_cu = 0;
-        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0), "occupancy");
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds_bytes), "occupancy");
         int64_t resident = (int64_t)std::max(1, per_cu) * s->n_cu;
         if (p->grid_fraction > 0.0f && p->grid_fraction < 1.0f)
             resident = std::max<int64_t>(1, (int64_t)((double)resident * p->grid_fraction));
@@ -1175,7 +1199,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
         }
         if (kern) {
             if (events) check(hipEventRecord(s->kev[0], q), "hipEventRecord");
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, q, dev, R, group_bits ? live[1] : live[0], cnt, cnt + 2, O,
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds_bytes, q, dev, R, group_bits ? live[1] : live[0], cnt, cnt + 2, O,
                                p->max_bounces, p->record_k, p->newton_mode, p->flags, ctr, MD, nbatch);
             if (events) check(hipEventRecord(s->kev[1], q), "hipEventRecord");
         }

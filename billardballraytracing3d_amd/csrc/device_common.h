@@ -16,8 +16,6 @@ namespace {
 constexpr int kStackCap = 48;
 constexpr int kStack4Cap = 64;                // 4-wide traversal stack (3 entries per level; depth <= 21 levels)
 constexpr double kPruneSlack = 1.000001;      // FP32 tmin prune: relative slack over the FP64 best_t
-constexpr int kCoopLanes = 16;                // stored scenes, leaf phase: at most this many lanes in it -> wave-cooperative stage 1
-constexpr int kCoopLanesLearned = 8;          // the same with the learned seed (measured best for each kernel)
 constexpr int kCompactGather = 24;            // compact scenes: leaf lanes a wave gathers before its pyramid walks
                                               // (10.4 -> 9.8 ms on 256 sub-scenes; 16-48 measure alike)
 constexpr int kDefaultWavesPerEu = 2;         // persistent-kernel occupancy target: 2 waves/EU keep the stored-scene
@@ -1047,8 +1045,8 @@ __device__ __forceinline__ Seed stage1_seed(const DevScene& S, int64_t k, const 
 // left in the leaf phase, a level costs one memory round trip instead of one
 // per hit box.
 template <int CM>
-__device__ Stage1 stage1_coop(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r, double widen,
-                              bool use_cull, uint32_t ms_in, uint32_t& n_cull, uint32_t& n_mt) {
+__device__ __forceinline__ Stage1 stage1_coop(const DevScene& S, int64_t k, const V3& o, const V3& d, const Ray32& r, double widen,
+                                              bool use_cull, uint32_t ms_in, uint32_t& n_cull, uint32_t& n_mt) {
     const int lane = __lane_id();
     const double* G = S.cxyz + (size_t)k * bbrt::kCoarseN * bbrt::kCoarseN * 3;
     const double* C = S.cell + (size_t)k * bbrt::kCells * bbrt::kCells * 4;
@@ -1150,6 +1148,198 @@ __device__ Stage1 stage1_coop(const DevScene& S, int64_t k, const V3& o, const V
         if (h2 && (!best.have || t2 < best.t || (t2 == best.t && k2 < best.key))) { best.t = t2; best.key = k2; best.have = true; }
     }
     return best;
+}
+
+// One owner's record for stage1_bfs, in LDS: its ray, its patch, its work
+// counts and its best (t, key) so far.
+struct BfsOwner {
+    double ox, oy, oz, dx, dy, dz;
+    double t;
+    float r32[6];
+    int64_t k;
+    uint32_t n_cull, n_mt;
+    int32_t key, have;
+};
+
+// Inclusive sum over the wave's lanes.
+__device__ __forceinline__ int wave_incl_sum(int v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// The lane whose entries (inclusive sums `incl`) hold list entry e: the number of
+// lanes whose entries all come before e.  Every lane calls it.
+__device__ __forceinline__ int wave_find(int incl, int e) {
+    int src = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(incl, src + step - 1) <= e) src += step;
+    return src;
+}
+
+// Stage 1 of every leaf lane's (ray, patch) pair at once, stored scenes with
+// culling (every lane calls it; `own` marks the lanes with a pair, `sup` their
+// hit super boxes, tested and counted by the caller).  The pyramid is walked
+// breadth-first over the whole wave.  A pass gives each lane one entry of the
+// current level, an (owner, parent box) pair, and the lane tests the parent's
+// whole group of child boxes for the owner's ray in one memory round trip: the
+// blocks of a hit super, the tiles of a hit block, the cells of a hit tile; the
+// hit children are the next level's entries, and each hit cell's six
+// Moller-Trumbore tests take one lane.  Entries stay in owner order, so a
+// segmented scan folds a pass's cell results per owner.  The boxes and cells
+// tested are the per-lane walk's and each owner's best is the minimum of
+// (t, key) over its cells, so the results and counts are the walk's; but a
+// level costs one round trip per 64 entries instead of one per hit box of each
+// lane.  `slots` is this wave's 64 owner records in LDS.
+__device__ __forceinline__ void stage1_bfs(const DevScene& S, bool own, int64_t k, const V3& o, const V3& d, const Ray32& r,
+                                           uint32_t sup, BfsOwner* slots, Stage1& best, uint32_t& n_cull, uint32_t& n_mt) {
+    const int lane = __lane_id();
+    if (own) {
+        BfsOwner& w = slots[lane];
+        w.ox = o.x; w.oy = o.y; w.oz = o.z; w.dx = d.x; w.dy = d.y; w.dz = d.z;
+        w.t = 0.0;
+        w.r32[0] = r.ox; w.r32[1] = r.oy; w.r32[2] = r.oz; w.r32[3] = r.ix; w.r32[4] = r.iy; w.r32[5] = r.iz;
+        w.k = k;
+        w.n_cull = 16u * (uint32_t)__popc(sup);   // the blocks of each hit super
+        w.n_mt = 0;
+        w.key = 0; w.have = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    auto ray_of = [&](int ow) -> Ray32 {
+        const BfsOwner& w = slots[ow];
+        return Ray32{ w.r32[0], w.r32[1], w.r32[2], w.r32[3], w.r32[4], w.r32[5] };
+    };
+    // level 1: (owner, hit super) -> the super's 16 block boxes
+    const int c1 = own ? __popc(sup) : 0;
+    const int i1 = wave_incl_sum(c1);
+    const int n1 = __shfl(i1, 63);
+    for (int p1 = 0; p1 < n1; p1 += 64) {
+        const int e1 = p1 + lane;
+        const int o1 = wave_find(i1, e1);
+        const uint32_t s1 = (uint32_t)__shfl((int)sup, o1);
+        const int b1 = __shfl(i1 - c1, o1);
+        const bool v1 = e1 < n1;
+        int sq = 0;
+        uint32_t m1 = 0;
+        if (v1) {
+            sq = nth_bit(s1, e1 - b1);
+            m1 = group_mask<16, bbrt::kG16>(S.box_block + (size_t)slots[o1].k * bbrt::kBlockFloats + sq * bbrt::kG16,
+                                            ray_of(o1));
+            if (m1) atomicAdd(&slots[o1].n_cull, 9u * (uint32_t)__popc(m1));   // the tiles of each hit block
+        }
+        // level 2: (owner, hit block) -> the block's 9 tile boxes
+        const int c2 = __popc(m1);
+        const int i2 = wave_incl_sum(c2);
+        const int n2 = __shfl(i2, 63);
+        for (int p2 = 0; p2 < n2; p2 += 64) {
+            const int e2 = p2 + lane;
+            const int src2 = wave_find(i2, e2);
+            const uint32_t mm1 = (uint32_t)__shfl((int)m1, src2);
+            const int b2 = __shfl(i2 - c2, src2);
+            const int sq2 = __shfl(sq, src2);
+            const int o2 = __shfl(o1, src2);
+            const bool v2 = e2 < n2;
+            int blk = 0;
+            uint32_t m2 = 0;
+            if (v2) {
+                const int bq = nth_bit(mm1, e2 - b2);
+                const int bi = bbrt::kBlocksPerSuper * (sq2 / 3) + (bq >> 2);
+                const int bj = bbrt::kBlocksPerSuper * (sq2 % 3) + (bq & 3);
+                blk = bi * bbrt::kBlockN + bj;
+                m2 = group_mask<9, bbrt::kG9>(S.box_tile + (size_t)slots[o2].k * bbrt::kTileFloats + blk * bbrt::kG9,
+                                              ray_of(o2));
+                if (m2) atomicAdd(&slots[o2].n_cull, 9u * (uint32_t)__popc(m2));   // the cells of each hit tile
+            }
+            // level 3: (owner, hit tile) -> the tile's 9 cell boxes
+            const int c3 = __popc(m2);
+            const int i3 = wave_incl_sum(c3);
+            const int n3 = __shfl(i3, 63);
+            for (int p3 = 0; p3 < n3; p3 += 64) {
+                const int e3 = p3 + lane;
+                const int src3 = wave_find(i3, e3);
+                const uint32_t mm2 = (uint32_t)__shfl((int)m2, src3);
+                const int b3 = __shfl(i3 - c3, src3);
+                const int blk3 = __shfl(blk, src3);
+                const int o3 = __shfl(o2, src3);
+                const bool v3 = e3 < n3;
+                int tile = 0;
+                uint32_t m3 = 0;
+                if (v3) {
+                    const int tq = nth_bit(mm2, e3 - b3);
+                    const int ti = 3 * (blk3 / bbrt::kBlockN) + tq / 3, tj = 3 * (blk3 % bbrt::kBlockN) + tq % 3;
+                    tile = ti * bbrt::kTileN + tj;
+                    m3 = group_mask<9, bbrt::kG9>(S.box_cell + (size_t)slots[o3].k * bbrt::kCellFloats +
+                                                  tile * bbrt::kG9, ray_of(o3));
+                    if (m3) atomicAdd(&slots[o3].n_mt, 6u * (uint32_t)__popc(m3));   // each hit cell's tests
+                }
+                // level 4: (owner, hit cell) -> its six Moller-Trumbore tests
+                const int c4 = __popc(m3);
+                const int i4 = wave_incl_sum(c4);
+                const int n4 = __shfl(i4, 63);
+                for (int p4 = 0; p4 < n4; p4 += 64) {
+                    const int e4 = p4 + lane;
+                    const int src4 = wave_find(i4, e4);
+                    const uint32_t mm3 = (uint32_t)__shfl((int)m3, src4);
+                    const int b4 = __shfl(i4 - c4, src4);
+                    const int tile4 = __shfl(tile, src4);
+                    const bool v4 = e4 < n4;
+                    const int o4s = __shfl(o3, src4);   // every lane takes part in the shuffle
+                    const int o4 = v4 ? o4s : -1;
+                    Stage1 ib{ 0.0, 0, false };
+                    if (v4) {
+                        const int cq = nth_bit(mm3, e4 - b4);
+                        const int ci = 3 * (tile4 / bbrt::kTileN) + cq / 3, cj = 3 * (tile4 % bbrt::kTileN) + cq % 3;
+                        const BfsOwner& w = slots[o4];
+                        const int64_t kk = w.k;
+                        const V3 oo = { w.ox, w.oy, w.oz }, dd = { w.dx, w.dy, w.dz };
+                        Counters dummy = {};
+                        cell_tests(S.cxyz + (size_t)kk * bbrt::kCoarseN * bbrt::kCoarseN * 3,
+                                   S.cell + (size_t)kk * bbrt::kCells * bbrt::kCells * 4, ci, cj, oo, dd, ib, dummy);
+                    }
+                    // per owner: the minimum (t, key) of its entries, at its last entry of the pass
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const double t2 = __shfl_up(ib.t, off);
+                        const int k2 = __shfl_up(ib.key, off);
+                        const bool h2 = __shfl_up(ib.have ? 1 : 0, off) != 0;
+                        const int ow2 = __shfl_up(o4, off);
+                        if (lane >= off && ow2 == o4 && h2 && (!ib.have || t2 < ib.t || (t2 == ib.t && k2 < ib.key))) {
+                            ib.t = t2; ib.key = k2; ib.have = true;
+                        }
+                    }
+                    const int next_ow = __shfl_down(o4, 1);
+                    if (v4 && ib.have && (lane == 63 || next_ow != o4)) {
+                        BfsOwner& w = slots[o4];
+                        if (!w.have || ib.t < w.t || (ib.t == w.t && ib.key < w.key)) {
+                            w.t = ib.t; w.key = ib.key; w.have = 1;
+                        }
+                    }
+                    // the next pass's owners read these records
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (own) {
+        const BfsOwner& w = slots[lane];
+        best = Stage1{ w.t, w.key, w.have != 0 };
+        n_cull = w.n_cull;
+        n_mt = w.n_mt;
+    }
+    // the records are written again by the next call: every lane's reads of this call come first
+    __builtin_amdgcn_wave_barrier();
 }
 
 // The triangle-centroid seed of a stage-1 hit (SurfacePatch.hpp:298-301).
@@ -1364,9 +1554,9 @@ __device__ int traverse(const DevScene& S, int32_t root_idx, double widen, const
 // face, or record + reflect + nudge.  Returns true while the ray continues;
 // o and d are advanced in place.  Patch indices are recorded relative to the
 // ray's sub-scene; hit_count is indexed by the global patch index.
-__device__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, int scn, V3& o, V3& d, int bounce,
-                              int max_bounces, int record_k, uint32_t flags, int err, int32_t err_obj,
-                              const PHit& best, int32_t best_obj, Counters& c) {
+__device__ __forceinline__ bool finish_bounce(const DevScene& S, const Outs& O, int32_t r, int scn, V3& o, V3& d, int bounce,
+                                              int max_bounces, int record_k, uint32_t flags, int err, int32_t err_obj,
+                                              const PHit& best, int32_t best_obj, Counters& c) {
     const bbrt::SceneInfo& SI = S.scenes[scn];
     if (err) {
         if (O.status) O.status[r] = bbrt::kError;
