@@ -230,15 +230,10 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         __syncthreads();
         lds = &mlp_lds;
     }
-    OwnerRay* owner_slots = nullptr;   // compact scenes: this wave's rays for the wave-wide cell tests
-    if constexpr (COMPACT) {
-        __shared__ OwnerRay owner_lds[256];
-        owner_slots = owner_lds + (threadIdx.x & ~63u);
-    }
-    BfsOwner* bfs_slots = nullptr;     // stored scenes: this wave's owner records for stage1_bfs (dynamic LDS,
-    if constexpr (!COMPACT && !FIRST) {   // 64 records a wave)
+    Owner* slots = nullptr;   // this wave's 64 owner records for the wave-wide stage 1 (dynamic LDS)
+    if constexpr (!FIRST) {
         extern __shared__ __align__(16) unsigned char bbrt_dyn_lds[];
-        bfs_slots = reinterpret_cast<BfsOwner*>(bbrt_dyn_lds) + (threadIdx.x & ~63u);
+        slots = reinterpret_cast<Owner*>(bbrt_dyn_lds) + (threadIdx.x & ~63u);
     }
     const int n_live = *n_live_p;
     const int lane = __lane_id();
@@ -438,110 +433,93 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 c.candidates++;
                 phase = kPhSeed;
             }
-        } else if constexpr (COMPACT) {
-            // compact scenes: each leaf lane walks its object's pyramid, and the cells the walks select are
-            // tested by the whole wave, 64 a pass; the walks resume until each lane's stage 1 hits (the lane
-            // parks for Newton) or its leaf ends.  Leaf lanes wait (the others traverse on) until
-            // kCompactGather of them are in a leaf or no other live lane has work before Newton, so the
-            // passes hold more cells; stage 1's outcome does not depend on when it runs.
-            const unsigned long long parked = __ballot(r >= 0 && phase == kPhSeed);
-            const bool go = (int)__popcll(in_leaf) >= kCompactGather || (in_leaf | parked) == __ballot(r >= 0);
-            bool walking = go && r >= 0 && phase == kPhLeaf;
-            CompactWalk w{ 0u, 0u, 0u, 0, 0, 0, 0, 0, false };
-            Stage1 s1{ 0.0, 0, false };
-            const double widen = S.scenes[scn].cull_widen;
-            if (walking) {
-                obj = S.leaf[leaf_start + q];
-                c.candidates++;
-                compact_walk_init(S, obj, r32, cull && S.meta[obj].cull, w, c);
-            }
-            while (__ballot(walking)) {
-                int ti = 0, tj = 0;
-                uint32_t mc = 0;
-                bool tile = false;
-                if (walking) {
-                    for (;;) {
-                        if (compact_walk_next(S, obj, r32, w, ti, tj, mc, c)) { tile = true; break; }
-                        // this object's walk is over: its stage-1 outcome (SurfacePatch.hpp:253-303)
-                        const Seed s = seed_of(s1);
-                        if (s.have) { sd = s; phase = kPhSeed; walking = false; break; }
-                        if (!next_object()) { walking = false; break; }
-                        obj = S.leaf[leaf_start + q];
-                        c.candidates++;
-                        s1 = Stage1{ 0.0, 0, false };
-                        compact_walk_init(S, obj, r32, cull && S.meta[obj].cull, w, c);
-                    }
-                }
-                compact_cells_wave(S, tile, obj, ti, tj, mc, o, d, r32, widen, w.use_cull, s1, c, owner_slots);
-            }
-        } else if (flags & BBRT_FLAG_STAGE1_LANE) {
-            // opt-in: each lane walks its own pyramids, object after object
+        } else if (!COMPACT && (flags & BBRT_FLAG_STAGE1_LANE)) {
+            // opt-in, stored scenes: each lane walks its own pyramids, object after object
             if (r >= 0 && phase == kPhLeaf) {
                 for (;;) {
                     obj = S.leaf[leaf_start + q];
-                    sd = stage1_seed<COMPACT ? 1 : 0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
+                    sd = stage1_seed<0>(S, obj, o, d, r32, cull, S.scenes[scn].cull_widen, c);
                     if (sd.have) { phase = kPhSeed; break; }   // parked until the wave runs Newton
                     if (!next_object()) break;
                 }
             }
-        } else if (!COMPACT && in_leaf) {
+        } else {
             // Every leaf lane first tests its object's nine super boxes itself, all in one round trip: an object
             // none of whose supers the ray enters has no stage-1 hit (counted as the walk counts it), and the lane
             // moves on.  Then the wave walks the pyramids below the hit supers of all its leaf lanes together,
-            // breadth-first (stage1_bfs); objects without culling, and every object under
-            // BBRT_FLAG_STAGE1_WAVE, are walked by the whole wave one lane at a time (stage1_coop).  The boxes,
-            // counts and results are the walk's own.
-            uint32_t sup = 0;
-            auto super_level = [&]() {   // from the lane's current entered object: the next one the walk must see
+            // breadth-first (stage1_bfs).  Objects without culling are walked by the whole wave one lane at a time
+            // (stored scenes, stage1_coop; also every object under BBRT_FLAG_STAGE1_WAVE), or tile by tile with
+            // the wave testing the cells (compact scenes).  The boxes, counts and results are the walk's own.
+            // Compact scenes: leaf lanes wait (the others traverse on) until kCompactGather of them are in a leaf
+            // or no other live lane has work before Newton, so the passes hold more entries; stage 1's outcome
+            // does not depend on when it runs.
+            bool go = in_leaf != 0;
+            if constexpr (COMPACT) {
+                const unsigned long long parked = __ballot(r >= 0 && phase == kPhSeed);
+                go = go && ((int)__popcll(in_leaf) >= kCompactGather || (in_leaf | parked) == __ballot(r >= 0));
+            }
+            if (go) {
+                uint32_t sup = 0;
+                auto super_level = [&]() {   // from the lane's current entered object: the next one the walk must see
+                    for (;;) {
+                        obj = S.leaf[leaf_start + q];
+                        if (!(cull && S.meta[obj].cull)) { sup = 0; return; }   // no culling: the walk does it
+                        sup = group_mask<9, bbrt::kG9>(S.box_super + (size_t)obj * bbrt::kSuperFloats, r32);
+                        c.cull_tests += 9;
+                        if (sup) return;
+                        c.candidates++;
+                        if (!next_object()) return;
+                    }
+                };
+                bool want_super = r >= 0 && phase == kPhLeaf;   // one call site: the lambda is inlined once
+                const bool serial_all = !COMPACT && (flags & BBRT_FLAG_STAGE1_WAVE) != 0;
                 for (;;) {
-                    obj = S.leaf[leaf_start + q];
-                    if (!(cull && S.meta[obj].cull)) { sup = 0; return; }   // brute force: the walk does it
-                    sup = group_mask<9, bbrt::kG9>(S.box_super + (size_t)obj * bbrt::kSuperFloats, r32);
-                    c.cull_tests += 9;
-                    if (sup) return;
-                    c.candidates++;
-                    if (!next_object()) return;
-                }
-            };
-            bool want_super = r >= 0 && phase == kPhLeaf;   // one call site: the lambda is inlined once
-            const bool serial_all = (flags & BBRT_FLAG_STAGE1_WAVE) != 0;
-            for (;;) {
-                if (want_super) { super_level(); want_super = false; }
-                const bool mine = r >= 0 && phase == kPhLeaf;
-                const unsigned long long need = __ballot(mine);
-                if (!need) break;
-                const unsigned long long serial = serial_all ? need : __ballot(mine && sup == 0);
-                if (!serial) {
+                    if (want_super) { super_level(); want_super = false; }
+                    const bool mine = r >= 0 && phase == kPhLeaf;
+                    const unsigned long long need = __ballot(mine);
+                    if (!need) break;
+                    const unsigned long long serial = serial_all ? need : __ballot(mine && sup == 0);
                     Stage1 b1{ 0.0, 0, false };
                     uint32_t n_cull = 0, n_mt = 0;
-                    stage1_bfs(S, mine, obj, o, d, r32, sup, bfs_slots, b1, n_cull, n_mt);
-                    if (mine) {
+                    bool done = mine;   // the lanes whose object's stage 1 ends in this pass
+                    if (!serial) {
+                        stage1_bfs<COMPACT ? 1 : 0>(S, mine, obj, o, d, r32, S.scenes[scn].cull_widen, sup, slots, b1,
+                                                    n_cull, n_mt);
+                    } else if constexpr (COMPACT) {
+                        // objects without culling: each lane walks every tile, the wave tests the cells
+                        bool walking = mine && sup == 0;
+                        done = walking;
+                        CompactWalk w{ 0u, 0u, 0u, 0, 0, 0, 0, 0, false };
+                        if (walking) compact_walk_init(S, obj, r32, false, w, c);
+                        while (__ballot(walking)) {
+                            int ti = 0, tj = 0;
+                            uint32_t mc = 0;
+                            bool tile = false;
+                            if (walking) { tile = compact_walk_next(S, obj, r32, w, ti, tj, mc, c); walking = tile; }
+                            compact_cells_wave(S, tile, obj, ti, tj, mc, o, d, r32, S.scenes[scn].cull_widen, false,
+                                               b1, c, slots);
+                        }
+                    } else {
+                        const int ld = __ffsll((long long)serial) - 1;
+                        const int32_t kk = __shfl(obj, ld);
+                        const uint32_t lsup = (uint32_t)__shfl((int)sup, ld);
+                        const V3 lo_ = { __shfl(o.x, ld), __shfl(o.y, ld), __shfl(o.z, ld) };
+                        const V3 ld_ = { __shfl(d.x, ld), __shfl(d.y, ld), __shfl(d.z, ld) };
+                        const Ray32 lr = { __shfl(r32.ox, ld), __shfl(r32.oy, ld), __shfl(r32.oz, ld),
+                                           __shfl(r32.ix, ld), __shfl(r32.iy, ld), __shfl(r32.iz, ld) };
+                        const int lscn = __shfl(scn, ld);
+                        const bool lcull = __shfl(cull ? 1 : 0, ld) != 0;
+                        b1 = stage1_coop<0>(S, kk, lo_, ld_, lr, S.scenes[lscn].cull_widen, lcull && S.meta[kk].cull,
+                                            lsup, n_cull, n_mt);
+                        done = lane == ld;
+                    }
+                    if (done) {
                         c.candidates++;
                         c.cull_tests += n_cull;
                         c.mt_tests += n_mt;
                         if (b1.have) { sd = seed_of(b1); phase = kPhSeed; }   // parked until the wave runs Newton
                         else if (next_object()) want_super = true;
                     }
-                    continue;
-                }
-                const int ld = __ffsll((long long)serial) - 1;
-                const int32_t kk = __shfl(obj, ld);
-                const uint32_t lsup = (uint32_t)__shfl((int)sup, ld);
-                const V3 lo_ = { __shfl(o.x, ld), __shfl(o.y, ld), __shfl(o.z, ld) };
-                const V3 ld_ = { __shfl(d.x, ld), __shfl(d.y, ld), __shfl(d.z, ld) };
-                const Ray32 lr = { __shfl(r32.ox, ld), __shfl(r32.oy, ld), __shfl(r32.oz, ld),
-                                   __shfl(r32.ix, ld), __shfl(r32.iy, ld), __shfl(r32.iz, ld) };
-                const int lscn = __shfl(scn, ld);
-                const bool lcull = __shfl(cull ? 1 : 0, ld) != 0;
-                uint32_t n_cull = 0, n_mt = 0;
-                const Stage1 b1 = stage1_coop<COMPACT ? 1 : 0>(S, kk, lo_, ld_, lr, S.scenes[lscn].cull_widen,
-                                                               lcull && S.meta[kk].cull, lsup, n_cull, n_mt);
-                if (lane == ld) {
-                    c.candidates++;
-                    c.cull_tests += n_cull;
-                    c.mt_tests += n_mt;
-                    if (b1.have) { sd = seed_of(b1); phase = kPhSeed; }   // parked until the wave runs Newton
-                    else if (next_object()) want_super = true;
                 }
             }
         }
@@ -1170,7 +1148,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
             kern = wpe == 2 ? &k_persist<2, false, false, false> : (wpe == 3 ? &k_persist<3, false, false, false>
                                                                               : &k_persist<4, false, false, false>);
         }
-        lds_bytes = (!s->compact && !first) ? (size_t)block * sizeof(BfsOwner) : 0;
+        lds_bytes = first ? 0 : (size_t)block * sizeof(Owner);
         int per_cu = 0;
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds_bytes), "occupancy");
         int64_t resident = (int64_t)std::max(1, per_cu) * s->n_cu;

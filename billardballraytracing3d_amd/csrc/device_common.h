@@ -634,14 +634,19 @@ __device__ __forceinline__ int nth_bit(unsigned long long m, int k) {
     return lo;
 }
 
-// A lane's ray as the other lanes of its wave read it during the wave-wide
-// cell tests (one LDS slot per thread of the block).
-struct OwnerRay {
+// A lane's stage-1 record as the other lanes of its wave read it while they
+// test boxes and cells for it (one LDS record per thread of the block): its
+// ray, its patch, the culling widening of its scene, its work counts and its
+// best (t, key) so far.
+struct Owner {
     double ox, oy, oz, dx, dy, dz;
+    double t;
     double widen;
     float r32[6];
-    int32_t use_cull;
-    int32_t pad;
+    int64_t k;
+    uint32_t n_cull, n_mt;
+    int32_t key, have;
+    int32_t use_cull, pad;
 };
 
 // The cells the lanes' walks selected, tested by the whole wave (every lane
@@ -657,7 +662,7 @@ struct OwnerRay {
 __device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have, int32_t k, int ti, int tj,
                                                    uint32_t mc, const V3& o, const V3& d, const Ray32& r,
                                                    double widen, bool use_cull, Stage1& best, Counters& c,
-                                                   OwnerRay* slots) {
+                                                   Owner* slots) {
     const int lane = __lane_id();
     const int cnt = have ? __popc(mc) : 0;
     int incl = cnt;
@@ -670,7 +675,7 @@ __device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have,
     if (total == 0) return;
     const int start = incl - cnt;
     if (have) {
-        OwnerRay& me = slots[lane];
+        Owner& me = slots[lane];
         me.ox = o.x; me.oy = o.y; me.oz = o.z; me.dx = d.x; me.dy = d.y; me.dz = d.z;
         me.widen = widen;
         me.r32[0] = r.ox; me.r32[1] = r.oy; me.r32[2] = r.oz; me.r32[3] = r.ix; me.r32[4] = r.iy; me.r32[5] = r.iz;
@@ -696,7 +701,7 @@ __device__ __forceinline__ void compact_cells_wave(const DevScene& S, bool have,
             const CellData cd = compact_cell_any(S, kk, ci, cj);
             // the owner's ray only now: it is not held across the cell's field loads
             __builtin_amdgcn_sched_barrier(0);
-            const OwnerRay& R = slots[own];
+            const Owner& R = slots[own];
             const Ray32 orr = { R.r32[0], R.r32[1], R.r32[2], R.r32[3], R.r32[4], R.r32[5] };
             if (!R.use_cull || compact_cell_box(cd, R.widen, orr)) {
                 const V3 oo = { R.ox, R.oy, R.oz }, od = { R.dx, R.dy, R.dz };
@@ -1150,17 +1155,6 @@ __device__ __forceinline__ Stage1 stage1_coop(const DevScene& S, int64_t k, cons
     return best;
 }
 
-// One owner's record for stage1_bfs, in LDS: its ray, its patch, its work
-// counts and its best (t, key) so far.
-struct BfsOwner {
-    double ox, oy, oz, dx, dy, dz;
-    double t;
-    float r32[6];
-    int64_t k;
-    uint32_t n_cull, n_mt;
-    int32_t key, have;
-};
-
 // Inclusive sum over the wave's lanes.
 __device__ __forceinline__ int wave_incl_sum(int v) {
     const int lane = __lane_id();
@@ -1182,29 +1176,35 @@ __device__ __forceinline__ int wave_find(int incl, int e) {
     return src;
 }
 
-// Stage 1 of every leaf lane's (ray, patch) pair at once, stored scenes with
-// culling (every lane calls it; `own` marks the lanes with a pair, `sup` their
-// hit super boxes, tested and counted by the caller).  The pyramid is walked
+// Stage 1 of every leaf lane's (ray, patch) pair at once, with culling (every
+// lane calls it; `own` marks the lanes with a pair, `sup` their hit super
+// boxes, tested and counted by the caller).  The pyramid is walked
 // breadth-first over the whole wave.  A pass gives each lane one entry of the
 // current level, an (owner, parent box) pair, and the lane tests the parent's
 // whole group of child boxes for the owner's ray in one memory round trip: the
-// blocks of a hit super, the tiles of a hit block, the cells of a hit tile; the
-// hit children are the next level's entries, and each hit cell's six
-// Moller-Trumbore tests take one lane.  Entries stay in owner order, so a
-// segmented scan folds a pass's cell results per owner.  The boxes and cells
-// tested are the per-lane walk's and each owner's best is the minimum of
-// (t, key) over its cells, so the results and counts are the walk's; but a
-// level costs one round trip per 64 entries instead of one per hit box of each
-// lane.  `slots` is this wave's 64 owner records in LDS.
+// blocks of a hit super, the tiles of a hit block, the cells of a hit tile (CM
+// 1, compact scenes: the tile's quantized cell boxes); the hit children are
+// the next level's entries, and each hit cell's six Moller-Trumbore tests take
+// one lane (CM 1: the cell evaluated from the fields, its exact box tested
+// first).  Entries stay in owner order, so a segmented scan folds a pass's cell
+// results per owner.  The boxes and cells tested are the per-lane walk's and
+// each owner's best is the minimum of (t, key) over its cells, so the results
+// and counts are the walk's; but a level costs one round trip per 64 entries
+// instead of one per hit box of each lane.  `slots` is this wave's 64 owner
+// records in LDS.
+template <int CM>
 __device__ __forceinline__ void stage1_bfs(const DevScene& S, bool own, int64_t k, const V3& o, const V3& d, const Ray32& r,
-                                           uint32_t sup, BfsOwner* slots, Stage1& best, uint32_t& n_cull, uint32_t& n_mt) {
+                                           double widen, uint32_t sup, Owner* slots, Stage1& best, uint32_t& n_cull,
+                                           uint32_t& n_mt) {
     const int lane = __lane_id();
     if (own) {
-        BfsOwner& w = slots[lane];
+        Owner& w = slots[lane];
         w.ox = o.x; w.oy = o.y; w.oz = o.z; w.dx = d.x; w.dy = d.y; w.dz = d.z;
         w.t = 0.0;
+        w.widen = widen;
         w.r32[0] = r.ox; w.r32[1] = r.oy; w.r32[2] = r.oz; w.r32[3] = r.ix; w.r32[4] = r.iy; w.r32[5] = r.iz;
         w.k = k;
+        w.use_cull = 1;
         w.n_cull = 16u * (uint32_t)__popc(sup);   // the blocks of each hit super
         w.n_mt = 0;
         w.key = 0; w.have = 0;
@@ -1213,7 +1213,7 @@ __device__ __forceinline__ void stage1_bfs(const DevScene& S, bool own, int64_t 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     auto ray_of = [&](int ow) -> Ray32 {
-        const BfsOwner& w = slots[ow];
+        const Owner& w = slots[ow];
         return Ray32{ w.r32[0], w.r32[1], w.r32[2], w.r32[3], w.r32[4], w.r32[5] };
     };
     // level 1: (owner, hit super) -> the super's 16 block boxes
@@ -1275,9 +1275,15 @@ __device__ __forceinline__ void stage1_bfs(const DevScene& S, bool own, int64_t 
                     const int tq = nth_bit(mm2, e3 - b3);
                     const int ti = 3 * (blk3 / bbrt::kBlockN) + tq / 3, tj = 3 * (blk3 % bbrt::kBlockN) + tq % 3;
                     tile = ti * bbrt::kTileN + tj;
-                    m3 = group_mask<9, bbrt::kG9>(S.box_cell + (size_t)slots[o3].k * bbrt::kCellFloats +
-                                                  tile * bbrt::kG9, ray_of(o3));
-                    if (m3) atomicAdd(&slots[o3].n_mt, 6u * (uint32_t)__popc(m3));   // each hit cell's tests
+                    const int64_t k3 = slots[o3].k;
+                    if constexpr (CM == 1) {
+                        m3 = qcell_mask(S, k3, S.box_tile + (size_t)k3 * bbrt::kTileFloats + blk3 * bbrt::kG9 + 6 * tq,
+                                        ti, tj, ray_of(o3));
+                    } else {
+                        m3 = group_mask<9, bbrt::kG9>(S.box_cell + (size_t)k3 * bbrt::kCellFloats + tile * bbrt::kG9,
+                                                      ray_of(o3));
+                        if (m3) atomicAdd(&slots[o3].n_mt, 6u * (uint32_t)__popc(m3));   // each hit cell's tests
+                    }
                 }
                 // level 4: (owner, hit cell) -> its six Moller-Trumbore tests
                 const int c4 = __popc(m3);
@@ -1296,12 +1302,24 @@ __device__ __forceinline__ void stage1_bfs(const DevScene& S, bool own, int64_t 
                     if (v4) {
                         const int cq = nth_bit(mm3, e4 - b4);
                         const int ci = 3 * (tile4 / bbrt::kTileN) + cq / 3, cj = 3 * (tile4 % bbrt::kTileN) + cq % 3;
-                        const BfsOwner& w = slots[o4];
-                        const int64_t kk = w.k;
-                        const V3 oo = { w.ox, w.oy, w.oz }, dd = { w.dx, w.dy, w.dz };
+                        const int64_t kk = slots[o4].k;
                         Counters dummy = {};
-                        cell_tests(S.cxyz + (size_t)kk * bbrt::kCoarseN * bbrt::kCoarseN * 3,
-                                   S.cell + (size_t)kk * bbrt::kCells * bbrt::kCells * 4, ci, cj, oo, dd, ib, dummy);
+                        if constexpr (CM == 1) {
+                            const CellData cd = compact_cell_any(S, kk, ci, cj);
+                            // the owner's ray only now: it is not held across the cell's field loads
+                            __builtin_amdgcn_sched_barrier(0);
+                            const Owner& w = slots[o4];
+                            if (compact_cell_box(cd, w.widen, ray_of(o4))) {
+                                const V3 oo = { w.ox, w.oy, w.oz }, dd = { w.dx, w.dy, w.dz };
+                                cell_tests_data(cd, ci, cj, oo, dd, ib, dummy);
+                                atomicAdd(&slots[o4].n_mt, 6u);
+                            }
+                        } else {
+                            const Owner& w = slots[o4];
+                            const V3 oo = { w.ox, w.oy, w.oz }, dd = { w.dx, w.dy, w.dz };
+                            cell_tests(S.cxyz + (size_t)kk * bbrt::kCoarseN * bbrt::kCoarseN * 3,
+                                       S.cell + (size_t)kk * bbrt::kCells * bbrt::kCells * 4, ci, cj, oo, dd, ib, dummy);
+                        }
                     }
                     // per owner: the minimum (t, key) of its entries, at its last entry of the pass
 #pragma unroll
@@ -1316,7 +1334,7 @@ __device__ __forceinline__ void stage1_bfs(const DevScene& S, bool own, int64_t 
                     }
                     const int next_ow = __shfl_down(o4, 1);
                     if (v4 && ib.have && (lane == 63 || next_ow != o4)) {
-                        BfsOwner& w = slots[o4];
+                        Owner& w = slots[o4];
                         if (!w.have || ib.t < w.t || (ib.t == w.t && ib.key < w.key)) {
                             w.t = ib.t; w.key = ib.key; w.have = 1;
                         }
@@ -1333,7 +1351,7 @@ __device__ __forceinline__ void stage1_bfs(const DevScene& S, bool own, int64_t 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (own) {
-        const BfsOwner& w = slots[lane];
+        const Owner& w = slots[lane];
         best = Stage1{ w.t, w.key, w.have != 0 };
         n_cull = w.n_cull;
         n_mt = w.n_mt;
