@@ -450,14 +450,12 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
             // breadth-first (stage1_bfs).  Objects without culling are walked by the whole wave one lane at a time
             // (stored scenes, stage1_coop; also every object under BBRT_FLAG_STAGE1_WAVE), or tile by tile with
             // the wave testing the cells (compact scenes).  The boxes, counts and results are the walk's own.
-            // Compact scenes: leaf lanes wait (the others traverse on) until kCompactGather of them are in a leaf
-            // or no other live lane has work before Newton, so the passes hold more entries; stage 1's outcome
-            // does not depend on when it runs.
-            bool go = in_leaf != 0;
-            if constexpr (COMPACT) {
-                const unsigned long long parked = __ballot(r >= 0 && phase == kPhSeed);
-                go = go && ((int)__popcll(in_leaf) >= kCompactGather || (in_leaf | parked) == __ballot(r >= 0));
-            }
+            // Leaf lanes wait (the others traverse on) until kStage1Gather of them are in a leaf or no other
+            // live lane has work before Newton, so the passes hold more entries; stage 1's outcome does not
+            // depend on when it runs.
+            const unsigned long long parked = __ballot(r >= 0 && phase == kPhSeed);
+            const bool go = in_leaf != 0 &&
+                            ((int)__popcll(in_leaf) >= kStage1Gather || (in_leaf | parked) == __ballot(r >= 0));
             if (go) {
                 uint32_t sup = 0;
                 auto super_level = [&]() {   // from the lane's current entered object: the next one the walk must see

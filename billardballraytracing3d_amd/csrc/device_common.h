@@ -16,8 +16,10 @@ namespace {
 constexpr int kStackCap = 48;
 constexpr int kStack4Cap = 64;                // 4-wide traversal stack (3 entries per level; depth <= 21 levels)
 constexpr double kPruneSlack = 1.000001;      // FP32 tmin prune: relative slack over the FP64 best_t
-constexpr int kCompactGather = 24;            // compact scenes: leaf lanes a wave gathers before its pyramid walks
-                                              // (10.4 -> 9.8 ms on 256 sub-scenes; 16-48 measure alike)
+constexpr int kStage1Gather = 24;             // leaf lanes a wave gathers before its stage-1 walk: one trace of the
+                                              // Sinai scene 1.45 -> 1.33 ms (four in flight: unchanged; 48 gains
+                                              // 4 % more alone but loses 2 % in flight), 256 compact sub-scenes
+                                              // 5.75 -> 5.51 ms
 constexpr int kDefaultWavesPerEu = 2;         // persistent-kernel occupancy target: 2 waves/EU keep the stored-scene
                                               // kernel spill-free and measured 2-4 % faster than 3 on MI355X
 constexpr int kDefaultWavesPerEuCompact = 2;  // compact scenes: the on-the-fly cell evaluation wants the registers
