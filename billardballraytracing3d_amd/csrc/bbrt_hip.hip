@@ -329,11 +329,20 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                     const double2 b01 = lb2[0], b23 = lb2[1], b45 = lb2[2];
                     uint32_t en = 0;
                     if (cull) {
-                        for (int k = 0; k < lc; ++k) {
-                            const float4* b4 = reinterpret_cast<const float4*>(S.leaf_pbox + (size_t)(ls + k) * 8);
-                            const float4 a = b4[0], bb = b4[1];
-                            const float bx[6] = { a.x, a.y, a.z, a.w, bb.x, bb.y };
-                            if (box32(bx, r32)) en |= 1u << k;
+                        // four objects' boxes a round trip (their loads go out together)
+                        const float4* b4 = reinterpret_cast<const float4*>(S.leaf_pbox + (size_t)ls * 8);
+                        for (int k0 = 0; k0 < lc; k0 += 4) {
+                            float4 a[4], bb[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                if (k0 + u < lc) { a[u] = b4[2 * (k0 + u)]; bb[u] = b4[2 * (k0 + u) + 1]; }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                if (k0 + u < lc) {
+                                    const float bx[6] = { a[u].x, a[u].y, a[u].z, a[u].w, bb[u].x, bb[u].y };
+                                    if (box32(bx, r32)) en |= 1u << (k0 + u);
+                                }
+                            }
                         }
                     } else {
                         en = (1u << lc) - 1u;
