@@ -75,11 +75,12 @@ enum { BBRT_FLAG_NO_CULL = 1u,          /* stage 1 as the reference's brute forc
        BBRT_FLAG_GRAPH = 16u,           /* capture the trace (counter fills, k_init, persistent kernel) in a
                                            hipGraph on the handle's stream and replay it while the arguments
                                            are unchanged; kernel_ms[0] is then the whole graph's time */
-       BBRT_FLAG_STAGE1_WAVE = 32u,     /* persistent kernel, stored scenes: every stage 1 wave-cooperative */
-       BBRT_FLAG_STAGE1_LANE = 64u,     /* ... every stage 1 per lane (default: wave-cooperative when at most
-                                           16 lanes of a wave need one, 8 with the learned seed); results and
-                                           counters do not change.  Compact scenes ignore both: each lane
-                                           walks its own pyramid and the wave tests the selected cells */
+       BBRT_FLAG_STAGE1_WAVE = 32u,     /* persistent kernel, stored scenes: every stage 1 walked by the whole
+                                           wave, one lane's (ray, patch) pair at a time */
+       BBRT_FLAG_STAGE1_LANE = 64u,     /* ... every stage 1 walked by its own lane (default: the wave walks all
+                                           its leaf lanes' pyramids together, breadth-first, one box group or
+                                           cell a lane per pass); results and counters do not change.
+                                           Compact scenes ignore both */
        BBRT_FLAG_SEED_FIRST = 256u,     /* learned seed, network first (opt-in, parity measured, not promised):
                                            each candidate patch goes to Newton from the network's seed without
                                            stage 1.  A root that converges inside the patch (classification and
