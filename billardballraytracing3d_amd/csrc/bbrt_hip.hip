@@ -313,14 +313,73 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         // culling limit test the slots' binary nodes in FP64 instead.
         if (r >= 0 && phase == kPhTrav) {
             for (;;) {
-                if (cur == -1 || (double)cur_t > best_t * kPruneSlack) {
-                    if (sp == 0) { phase = kPhDone; break; }
-                    --sp;
-                    cur = stack[sp].ref; cur_t = stack[sp].tmin;
-                    continue;
+                // inner nodes and pops until the lane reaches a leaf or its walk ends; the lanes at a leaf wait
+                // and test their leaves together below (while-while: a step runs one of the two paths, not both)
+                for (;;) {
+                    if (cur == -1 || (double)cur_t > best_t * kPruneSlack) {
+                        if (sp == 0) { phase = kPhDone; break; }
+                        --sp;
+                        cur = stack[sp].ref; cur_t = stack[sp].tmin;
+                        continue;
+                    }
+                    if (cur <= -2) break;
+                    c.bvh_tests++;
+                    const bbrt::Bvh4Node* N = S.bvh4 + cur;
+                    const int4 rf = *reinterpret_cast<const int4*>(N->ref);
+                    const int axes = N->axes;
+                    const int32_t ref[4] = { rf.x, rf.y, rf.z, rf.w };
+                    float tmn[4];
+                    uint32_t hm;
+                    if (cull) {
+                        hm = slots32(N, r32, tmn);
+                    } else {
+                        hm = 0;
+#pragma unroll
+                        for (int sl = 0; sl < 4; ++sl) {
+                            tmn[sl] = 0.0f;
+                            if (ref[sl] != -1) {
+                                double tm = 0.0;
+                                const bool hit = ref[sl] >= 0
+                                    ? box_static(o, inv, S.bvh[ref[sl]], tm)
+                                    : box_static6(o, inv, S.leaf_box + 6 * (size_t)((-2 - ref[sl]) >> 3),
+                                                  S.leaf_box + 6 * (size_t)((-2 - ref[sl]) >> 3) + 3, tm);
+                                if (hit) {
+                                    hm |= 1u << sl;
+                                    tmn[sl] = (float)tm;
+                                }
+                            }
+                        }
+                    }
+                    hm &= (ref[0] != -1 ? 1u : 0u) | (ref[1] != -1 ? 2u : 0u) | (ref[2] != -1 ? 4u : 0u) |
+                          (ref[3] != -1 ? 8u : 0u);
+                    // depth-first order of the slots: the node's split axis picks the pair, each
+                    // child's axis the slot within it (BVH.hpp:118-125)
+                    const int f0 = (negm >> (axes & 3)) & 1, fl = (negm >> ((axes >> 2) & 3)) & 1,
+                              fr = (negm >> ((axes >> 4) & 3)) & 1;
+                    const int l0 = fl, l1 = 1 - fl, r0 = 2 + fr, r1 = 3 - fr;
+                    const int sq[4] = { f0 ? r0 : l0, f0 ? r1 : l1, f0 ? l0 : r0, f0 ? l1 : r1 };
+                    int32_t R[4];
+                    float T[4];
+                    uint32_t pm = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int sl = sq[k];
+                        R[k] = sl == 0 ? ref[0] : (sl == 1 ? ref[1] : (sl == 2 ? ref[2] : ref[3]));
+                        T[k] = sl == 0 ? tmn[0] : (sl == 1 ? tmn[1] : (sl == 2 ? tmn[2] : tmn[3]));
+                        pm |= ((hm >> sl) & 1u) << k;
+                    }
+                    if (pm == 0) { cur = -1; continue; }
+                    if (sp + 3 > kStack4Cap) { err = bbrt::kErrStack; phase = kPhDone; break; }
+                    const int fq = __ffs(pm) - 1;
+#pragma unroll
+                    for (int k = 3; k >= 1; --k)
+                        if (k > fq && ((pm >> k) & 1u)) { stack[sp].ref = R[k]; stack[sp].tmin = T[k]; ++sp; }
+                    cur = fq == 0 ? R[0] : (fq == 1 ? R[1] : (fq == 2 ? R[2] : R[3]));
+                    cur_t = fq == 0 ? T[0] : (fq == 1 ? T[1] : (fq == 2 ? T[2] : T[3]));
                 }
+                if (phase != kPhTrav) break;
                 c.bvh_tests++;
-                if (cur <= -2) {
+                {
                     // a leaf: its FP64 box, object ids and culling boxes are addressed by its start,
                     // so the loads go out together
                     const int v = -2 - cur;
@@ -365,58 +424,6 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                     cur = -1;
                     continue;
                 }
-                const bbrt::Bvh4Node* N = S.bvh4 + cur;
-                const int4 rf = *reinterpret_cast<const int4*>(N->ref);
-                const int axes = N->axes;
-                const int32_t ref[4] = { rf.x, rf.y, rf.z, rf.w };
-                float tmn[4];
-                uint32_t hm;
-                if (cull) {
-                    hm = slots32(N, r32, tmn);
-                } else {
-                    hm = 0;
-#pragma unroll
-                    for (int sl = 0; sl < 4; ++sl) {
-                        tmn[sl] = 0.0f;
-                        if (ref[sl] != -1) {
-                            double tm = 0.0;
-                            const bool hit = ref[sl] >= 0
-                                ? box_static(o, inv, S.bvh[ref[sl]], tm)
-                                : box_static6(o, inv, S.leaf_box + 6 * (size_t)((-2 - ref[sl]) >> 3),
-                                              S.leaf_box + 6 * (size_t)((-2 - ref[sl]) >> 3) + 3, tm);
-                            if (hit) {
-                                hm |= 1u << sl;
-                                tmn[sl] = (float)tm;
-                            }
-                        }
-                    }
-                }
-                hm &= (ref[0] != -1 ? 1u : 0u) | (ref[1] != -1 ? 2u : 0u) | (ref[2] != -1 ? 4u : 0u) |
-                      (ref[3] != -1 ? 8u : 0u);
-                // depth-first order of the slots: the node's split axis picks the pair, each
-                // child's axis the slot within it (BVH.hpp:118-125)
-                const int f0 = (negm >> (axes & 3)) & 1, fl = (negm >> ((axes >> 2) & 3)) & 1,
-                          fr = (negm >> ((axes >> 4) & 3)) & 1;
-                const int l0 = fl, l1 = 1 - fl, r0 = 2 + fr, r1 = 3 - fr;
-                const int sq[4] = { f0 ? r0 : l0, f0 ? r1 : l1, f0 ? l0 : r0, f0 ? l1 : r1 };
-                int32_t R[4];
-                float T[4];
-                uint32_t pm = 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int sl = sq[k];
-                    R[k] = sl == 0 ? ref[0] : (sl == 1 ? ref[1] : (sl == 2 ? ref[2] : ref[3]));
-                    T[k] = sl == 0 ? tmn[0] : (sl == 1 ? tmn[1] : (sl == 2 ? tmn[2] : tmn[3]));
-                    pm |= ((hm >> sl) & 1u) << k;
-                }
-                if (pm == 0) { cur = -1; continue; }
-                if (sp + 3 > kStack4Cap) { err = bbrt::kErrStack; phase = kPhDone; break; }
-                const int fq = __ffs(pm) - 1;
-#pragma unroll
-                for (int k = 3; k >= 1; --k)
-                    if (k > fq && ((pm >> k) & 1u)) { stack[sp].ref = R[k]; stack[sp].tmin = T[k]; ++sp; }
-                cur = fq == 0 ? R[0] : (fq == 1 ? R[1] : (fq == 2 ? R[2] : R[3]));
-                cur_t = fq == 0 ? T[0] : (fq == 1 ? T[1] : (fq == 2 ? T[2] : T[3]));
             }
         }
 
