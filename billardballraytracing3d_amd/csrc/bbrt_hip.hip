@@ -354,20 +354,19 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                           (ref[3] != -1 ? 8u : 0u);
                     // depth-first order of the slots: the node's split axis picks the pair, each
                     // child's axis the slot within it (BVH.hpp:118-125)
-                    const int f0 = (negm >> (axes & 3)) & 1, fl = (negm >> ((axes >> 2) & 3)) & 1,
-                              fr = (negm >> ((axes >> 4) & 3)) & 1;
-                    const int l0 = fl, l1 = 1 - fl, r0 = 2 + fr, r1 = 3 - fr;
-                    const int sq[4] = { f0 ? r0 : l0, f0 ? r1 : l1, f0 ? l0 : r0, f0 ? l1 : r1 };
-                    int32_t R[4];
-                    float T[4];
-                    uint32_t pm = 0;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const int sl = sq[k];
-                        R[k] = sl == 0 ? ref[0] : (sl == 1 ? ref[1] : (sl == 2 ? ref[2] : ref[3]));
-                        T[k] = sl == 0 ? tmn[0] : (sl == 1 ? tmn[1] : (sl == 2 ? tmn[2] : tmn[3]));
-                        pm |= ((hm >> sl) & 1u) << k;
-                    }
+                    // (slots 0-1 are the left pair, 2-3 the right one; conditional swaps order each pair, then
+                    // the pairs)
+                    const bool f0 = ((negm >> (axes & 3)) & 1u) != 0, fl = ((negm >> ((axes >> 2) & 3)) & 1u) != 0,
+                               fr = ((negm >> ((axes >> 4) & 3)) & 1u) != 0;
+                    const int32_t a0 = fl ? ref[1] : ref[0], a1 = fl ? ref[0] : ref[1];
+                    const int32_t b0 = fr ? ref[3] : ref[2], b1 = fr ? ref[2] : ref[3];
+                    const float ta0 = fl ? tmn[1] : tmn[0], ta1 = fl ? tmn[0] : tmn[1];
+                    const float tb0 = fr ? tmn[3] : tmn[2], tb1 = fr ? tmn[2] : tmn[3];
+                    const uint32_t ma = fl ? (((hm >> 1) & 1u) | ((hm & 1u) << 1)) : (hm & 3u);
+                    const uint32_t mb = fr ? (((hm >> 3) & 1u) | (((hm >> 2) & 1u) << 1)) : ((hm >> 2) & 3u);
+                    const int32_t R[4] = { f0 ? b0 : a0, f0 ? b1 : a1, f0 ? a0 : b0, f0 ? a1 : b1 };
+                    const float T[4] = { f0 ? tb0 : ta0, f0 ? tb1 : ta1, f0 ? ta0 : tb0, f0 ? ta1 : tb1 };
+                    const uint32_t pm = f0 ? (mb | (ma << 2)) : (ma | (mb << 2));
                     if (pm == 0) { cur = -1; continue; }
                     if (sp + 3 > kStack4Cap) { err = bbrt::kErrStack; phase = kPhDone; break; }
                     const int fq = __ffs(pm) - 1;
