@@ -252,6 +252,9 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
     float cur_t = 0.0f;                       // its entry t (FP32, prune only)
     uint32_t negm = 0;                        // direction sign bits: the depth-first order
     double best_t = INFINITY;
+    // the prune bound best_t * kPruneSlack rounded down to a float: for a float entry t,
+    // (double)t > best_t * kPruneSlack exactly when t > prune_f
+    float prune_f = INFINITY;
     PHit best{ 0, 0.0, { 0, 0, 0 }, { 0, 0, 0 }, 0 };
     int32_t best_obj = -1, err_obj = -1;
     int err = 0;
@@ -267,6 +270,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         r32 = { (float)o.x, (float)o.y, (float)o.z, 1.0f / (float)d.x, 1.0f / (float)d.y, 1.0f / (float)d.z };
         inv[0] = 1.0 / d.x; inv[1] = 1.0 / d.y; inv[2] = 1.0 / d.z;
         best_t = INFINITY;
+        prune_f = INFINITY;
         best_obj = -1; err_obj = -1; err = 0; sp = 0;
         negm = (inv[0] < 0.0 ? 1u : 0u) | (inv[1] < 0.0 ? 2u : 0u) | (inv[2] < 0.0 ? 4u : 0u);
         const int32_t ri = S.scenes[scn].root;
@@ -316,7 +320,7 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
                 // inner nodes and pops until the lane reaches a leaf or its walk ends; the lanes at a leaf wait
                 // and test their leaves together below (while-while: a step runs one of the two paths, not both)
                 for (;;) {
-                    if (cur == -1 || (double)cur_t > best_t * kPruneSlack) {
+                    if (cur == -1 || cur_t > prune_f) {
                         if (sp == 0) { phase = kPhDone; break; }
                         --sp;
                         cur = stack[sp].ref; cur_t = stack[sp].tmin;
@@ -598,7 +602,10 @@ k_persist(DevScene S, RayState R, const int32_t* __restrict__ live, const int32_
         }
         if (have_h) {
             if (h.status < 0) { err = -h.status; err_obj = obj; phase = kPhDone; }
-            else if (h.status == 1 && h.t < best_t) { best_t = h.t; best = h; best_obj = obj; }
+            else if (h.status == 1 && h.t < best_t) {
+                best_t = h.t; best = h; best_obj = obj;
+                prune_f = round_down_f(best_t * kPruneSlack);
+            }
             if (phase == kPhLeaf) next_object();   // the next entered object, or back to the walk
         }
 
