@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
-    ap.add_argument("--batch-in-flight", type=int, default=2,
+    ap.add_argument("--batch-in-flight", type=int, default=4,
                     help="batched leg: traces kept in flight (bbrt_trace_device_async); 1 = graph replays only")
     ap.add_argument("--batch-no-tune", action="store_true", help="batched leg: library default launch shape")
     ap.add_argument("--batch-eager", action="store_true",
