@@ -1125,7 +1125,7 @@ void trace_impl(bbrt_scene s, const double* d_o, const double* d_d, int64_t n, c
     unsigned grid = 1;
     size_t lds_bytes = 0;   // dynamic LDS of the persistent kernel
     const MlpDev MD{ s->mlp_frame, s->mlp_h1pre, static_cast<const u32x2_t*>(s->mlp_wfrag), s->mlp_params };
-    const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : kDefaultNewtonBatch;
+    const int nbatch = p->newton_batch > 0 ? std::min(p->newton_batch, 64) : default_newton_batch(s->compact != 0);
     if (!wavefront && p->max_bounces > 0) {
         const int wpe = p->waves_per_eu > 0 ? p->waves_per_eu : default_waves_per_eu(s->compact);
         const bool work = O.ray_work != nullptr;
@@ -1949,7 +1949,7 @@ int bbrt_autotune(bbrt_scene s, const double* origins, const double* dirs, const
         bbrt_params cur = *base;
         cur.flags &= ~(uint32_t)BBRT_FLAG_GRAPH;
         cur.block = default_block(base->seed_mode == BBRT_SEED_LEARNED, s->compact != 0);
-        cur.waves_per_eu = default_waves_per_eu(s->compact); cur.newton_batch = kDefaultNewtonBatch;
+        cur.waves_per_eu = default_waves_per_eu(s->compact); cur.newton_batch = default_newton_batch(s->compact != 0);
         cur.grid_fraction = 1.0f;
         double cur_ms = run(cur);
         auto sweep = [&](auto setter, std::initializer_list<double> values) {

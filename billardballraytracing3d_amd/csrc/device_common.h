@@ -29,6 +29,9 @@ inline int default_waves_per_eu(bool compact) { return compact ? kDefaultWavesPe
 // 256 for the learned seed (21 KB of LDS per block) and for compact scenes
 inline int default_block(bool learned, bool compact) { return (learned || compact) ? 256 : 64; }
 constexpr int kDefaultNewtonBatch = 64;       // parked lanes (stage-1 hits) that trigger a batched Newton (64: when no lane can go on)
+constexpr int kDefaultNewtonBatchCompact = 32;  // compact scenes: 32 (256 sub-scenes: 5.04-5.12 vs 5.53-5.60 ms at 64;
+                                                // the Sinai scene measures 32 and 64 alike)
+inline int default_newton_batch(bool compact) { return compact ? kDefaultNewtonBatchCompact : kDefaultNewtonBatch; }
 constexpr double kMtEps = 1e-9;               // MollerTrumbore.hpp:33
 constexpr double kOnNodeTol = 1.0e-12;        // PatchInterpolation.hpp:21
 constexpr int kNewtonMaxIters = 20;           // SurfacePatch.hpp:121

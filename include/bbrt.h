@@ -115,8 +115,8 @@ typedef struct {
                               (the compiler keeps registers within that occupancy, spilling if needed) */
     float grid_fraction;   /* persistent kernel: fraction of the resident workgroups launched (0 = all) */
     int32_t newton_batch;  /* persistent kernel: lanes of a wave whose stage 1 hit wait until this many can run
-                              Newton (and the learned seed's network) together (1..64; 0 = library default, 64);
-                              results do not depend on it */
+                              Newton (and the learned seed's network) together (1..64; 0 = library default:
+                              64, or 32 on a compact scene); results do not depend on it */
 } bbrt_params;
 
 /* Outputs; any pointer may be NULL.  Host memory for bbrt_trace, device
@@ -280,7 +280,7 @@ int bbrt_trace_device_noalloc(bbrt_scene scene, const double* origins, const dou
  * rays of a single scene; outputs in `out` are overwritten) under a coordinate
  * descent from the library defaults of the kernel the trace selects (block 64
  * for a stored scene with the triangle seed, else 256; 2 waves/EU; Newton
- * batch 64; the full grid): waves_per_eu {2, 3, 4}, newton_batch
+ * batch 64, 32 on a compact scene; the full grid): waves_per_eu {2, 3, 4}, newton_batch
  * {8, 16, 32, 64}, block {64, 128, 256}, grid_fraction {0.5, 0.75, 1}; each
  * candidate is the faster of two runs.  *best = *base with the fastest shape;
  * table (may be NULL) receives up to max_entries timed candidates in order. */
