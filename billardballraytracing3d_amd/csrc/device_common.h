@@ -21,7 +21,7 @@ constexpr int kStage1Gather = 24;             // leaf lanes a wave gathers befor
                                               // 4 % more alone but loses 2 % in flight), 256 compact sub-scenes
                                               // 5.75 -> 5.51 ms
 constexpr int kDefaultWavesPerEu = 2;         // persistent-kernel occupancy target: 2 waves/EU keep the stored-scene
-                                              // kernel spill-free and measured 2-4 % faster than 3 on MI355X
+                                              // kernel's spills small and measured 6-7 % faster than 3 on MI355X
 constexpr int kDefaultWavesPerEuCompact = 2;  // compact scenes: the on-the-fly cell evaluation wants the registers
 inline int default_waves_per_eu(bool compact) { return compact ? kDefaultWavesPerEuCompact : kDefaultWavesPerEu; }
 // threads per block of the persistent kernel: 64 for stored scenes with the triangle seed (as fast as 256
