@@ -25,6 +25,8 @@
 #include <cmath>
 #include <limits>
 #include <map>
+#include <string>
+#include <unordered_map>
 #include <stdexcept>
 #include <thread>
 
@@ -529,6 +531,10 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads, bool 
     S.n = n;
     S.meta.resize(n);
     int64_t node_total = 0;
+    // patches with the same node vectors (bit for bit) share one table of nodes and barycentric weights, so
+    // the lanes of a batched Newton read the same few cache lines
+    std::unordered_map<std::string, int64_t> node_tables;
+    std::string key;
     for (int64_t k = 0; k < n; ++k) {
         const int im = ps.imax[k], jm = ps.jmax[k];
         if (im < 2 || jm < 2 || im > kMaxNodes1D || jm > kMaxNodes1D)
@@ -538,10 +544,15 @@ void build_host_scene(const bbrt_patch_set& ps, HostScene& S, int threads, bool 
             throw std::runtime_error("bbrt_scene_create: inconsistent offsets at patch " + std::to_string(k));
         PatchMeta& M = S.meta[k];
         M.imax = im; M.jmax = jm;
-        M.node_off = node_total;
+        key.assign(reinterpret_cast<const char*>(&im), sizeof(im));
+        key.append(reinterpret_cast<const char*>(&jm), sizeof(jm));
+        key.append(reinterpret_cast<const char*>(ps.unodes + ps.u_off[k]), sizeof(double) * (size_t)im);
+        key.append(reinterpret_cast<const char*>(ps.vnodes + ps.v_off[k]), sizeof(double) * (size_t)jm);
+        const auto ins = node_tables.emplace(key, node_total);
+        M.node_off = ins.first->second;
+        if (ins.second) node_total += 2 * (im + jm);
         M.field_off = 12 * ps.f_off[k];
         M.max_sag = 0.0; M.cull = 0; M.nodeset = 0; M.pad1 = 0.0;
-        node_total += 2 * (im + jm);
     }
     S.nodes.resize(node_total);
     const int64_t field_total = 12 * ps.f_off[n];
